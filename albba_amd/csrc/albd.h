@@ -1,0 +1,153 @@
+/*
+ * albd.h -- the thin C ABI between libalbba's C host code and its HIP
+ * kernels.  Plain C: device memory appears as typed pointers, the stream and
+ * events stay inside albd_ctx.  Every launch function is asynchronous on the
+ * context's stream unless its comment says otherwise, and returns 0 or a
+ * negative ALBBA_E* code.
+ */
+#ifndef ALBD_H
+#define ALBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALBD_HIDDEN __attribute__((visibility("hidden")))
+#define ALBD_NSLOT 32 /* counter slots; blocks spread their atomics over them */
+
+typedef struct albd_ctx albd_ctx;
+
+/* Per-level counters, one copy per slot; the host sums the slots. */
+typedef struct albd_slot {
+    unsigned long long next_count;   /* vertices discovered this level        */
+    unsigned long long next_edges;   /* sum of out-degree of those vertices   */
+    unsigned long long next_words;   /* nonzero next-frontier words           */
+    unsigned long long scanned;      /* pull: column entries gathered         */
+    unsigned long long pad_[4];      /* one slot per 64 bytes                 */
+} albd_slot;
+
+typedef struct albd_counters {
+    albd_slot slot[ALBD_NSLOT];
+    unsigned long long queue_len;    /* current frontier queue length         */
+    unsigned long long qnext_len;    /* queue being built                     */
+    unsigned long long active_len;   /* active (unvisited) word list length   */
+    unsigned long long active_next;  /* list being built                      */
+    unsigned long long push_edges;   /* sum of degrees over the queue         */
+    unsigned long long misc[3];
+} albd_counters;
+
+/* Read-only graph arrays on the device. */
+typedef struct albd_graph {
+    int32_t n, C, pad, nslices, nwords;
+    int64_t npos, nnz, padded_nnz, nvops;
+    const int64_t *slice_ptr;   /* [nslices+1] */
+    const int32_t *slice_width; /* [nslices]   */
+    const int64_t *vop_ptr;     /* [nslices+1] */
+    const uint8_t *vop_len;     /* [nvops]     */
+    const int32_t *colidx;      /* [padded_nnz]*/
+    const int32_t *row_order;   /* [npos] position -> vertex id (-1 past n) */
+    const int32_t *row_len;     /* [npos]      */
+    const int64_t *push_ptr;    /* [npos+1]    */
+    const int32_t *push_col;    /* [nnz]       */
+    const uint64_t *haslen;     /* [nwords] bit set where row_len > 0 */
+} albd_graph;
+
+/* Mutable BFS state on the device. */
+typedef struct albd_state {
+    uint64_t *visited;          /* [nwords]  */
+    uint64_t *front;            /* [nwords]  current frontier bitmap */
+    uint64_t *next;             /* [nwords]  next frontier bitmap    */
+    uint64_t *summary;          /* [nwords/64+1] nonzero-word bits of front */
+    int32_t *queue;             /* [npos]    current frontier queue (positions) */
+    int32_t *queue_next;        /* [npos]    */
+    int64_t *prefix;            /* [npos+1]  degree prefix of the queue */
+    int64_t *block_sums;        /* [ALBD_SCAN_BLOCKS+1] */
+    uint32_t *pmin;             /* [npos]    push: min parent id per position */
+    int32_t *level;             /* [n]       vertex-id space */
+    int32_t *parent;            /* [n]       */
+    int32_t *active;            /* [nwords]  unvisited-word list */
+    int32_t *active_next;       /* [nwords]  */
+    albd_counters *ctr;         /* device counters */
+} albd_state;
+
+#define ALBD_SCAN_BLOCKS 1024
+
+/* ---- context / memory (synchronous unless noted) ---- */
+ALBD_HIDDEN int albd_device_count(int *count);
+ALBD_HIDDEN int albd_ctx_create(int device, albd_ctx **out);
+ALBD_HIDDEN void albd_ctx_destroy(albd_ctx *c);
+ALBD_HIDDEN int albd_ctx_device(const albd_ctx *c);
+ALBD_HIDDEN int albd_num_cus(const albd_ctx *c);
+ALBD_HIDDEN int albd_malloc(albd_ctx *c, size_t bytes, void **p);
+ALBD_HIDDEN void albd_free(albd_ctx *c, void *p);
+ALBD_HIDDEN int albd_h2d(albd_ctx *c, void *dst, const void *src, size_t bytes);
+ALBD_HIDDEN int albd_d2h(albd_ctx *c, void *dst, const void *src, size_t bytes);
+ALBD_HIDDEN int albd_d2d(albd_ctx *c, void *dst, const void *src, size_t bytes); /* async */
+ALBD_HIDDEN int albd_sync(albd_ctx *c);
+ALBD_HIDDEN const char *albd_error(void);
+/* pinned host mirror of the counters, filled by albd_read_counters */
+ALBD_HIDDEN albd_counters *albd_host_counters(albd_ctx *c);
+ALBD_HIDDEN int albd_read_counters(albd_ctx *c, const albd_state *st); /* sync */
+
+/* ---- events (timing on the context stream) ---- */
+#define ALBD_NEVENTS 8
+ALBD_HIDDEN int albd_event_record(albd_ctx *c, int idx);
+ALBD_HIDDEN int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms); /* syncs b */
+
+/* ---- fills (kernels, safe under graph capture) ---- */
+ALBD_HIDDEN int albd_fill_u32(albd_ctx *c, void *p, uint32_t v, size_t count);
+ALBD_HIDDEN int albd_fill_u64(albd_ctx *c, void *p, uint64_t v, size_t count);
+
+/* ---- BFS pieces ---- */
+/* haslen[w] from row_len */
+ALBD_HIDDEN int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen);
+/* reset every state array and seed the root (position rp, vertex id ro) */
+ALBD_HIDDEN int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                              int32_t rp, int32_t ro);
+/* zero st->next and the per-level slot counters */
+ALBD_HIDDEN int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_state *st);
+
+/* pull variants */
+enum { ALBD_PULL_ALL = 0, ALBD_PULL_LIST = 1, ALBD_PULL_LIST_CB = 2 };
+/* compact the unvisited-word list: from all words (from_list = 0) or from the
+ * previous list (from_list = 1); result in st->active, length in ctr->active_len */
+ALBD_HIDDEN int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st,
+                                  int from_list, int64_t prev_len);
+ALBD_HIDDEN int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st);
+ALBD_HIDDEN int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant,
+                          int32_t level, int64_t items_bound);
+ALBD_HIDDEN int albd_cb_supported(const albd_graph *g);
+
+/* push: queue from the front bitmap (when the last level was a pull) */
+ALBD_HIDDEN int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* push: prefix of degrees over the queue (qlen known on the host) */
+ALBD_HIDDEN int albd_push_scan(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                               int64_t qlen);
+ALBD_HIDDEN int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* push: merge next into visited, write level/parent from pmin, build queue_next */
+ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                                   int32_t level);
+
+/* ---- selector MLP (mlp.hip); every pointer is device memory ---- */
+#define ALBD_MLP_SCRATCH_PER_SAMPLE (32 + 32 + 4 + 32 + 32 + 1)
+/* out[ns*4] = MLP(x[ns*12]) */
+ALBD_HIDDEN int albd_mlp_forward(albd_ctx *c, const float *params, const float *x, int32_t ns,
+                                 float *out);
+/* loss[0] = mean squared error, grad[1604] = d loss / d params;
+ * scratch holds ns * ALBD_MLP_SCRATCH_PER_SAMPLE floats */
+ALBD_HIDDEN int albd_mlp_loss_grad(albd_ctx *c, const float *params, const float *x,
+                                   const float *t, int32_t ns, float *scratch, float *grad,
+                                   float *loss);
+ALBD_HIDDEN int albd_mlp_adam(albd_ctx *c, float *params, const float *grad, float *m, float *v,
+                              int32_t np, float lr, float b1, float b2, float eps, int32_t step);
+/* choice[0] = argmin over choices whose bit is set in valid_mask of MLP(feat[12]) */
+ALBD_HIDDEN int albd_mlp_decide(albd_ctx *c, const float *params, const float *feat,
+                                int32_t valid_mask, int32_t *choice);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,197 @@
+// Context, memory, events and fill kernels of the device layer.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "../albd.h"
+#include "../../../include/albba.h"
+#include "dev_common.h"
+
+static thread_local char g_err[512];
+
+const char *albd_error(void) { return g_err; }
+
+int albd_set_error(hipError_t e, const char *what, const char *file, int line)
+{
+    snprintf(g_err, sizeof g_err, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return ALBBA_EHIP;
+}
+
+int albd_set_error_msg(int code, const char *msg)
+{
+    snprintf(g_err, sizeof g_err, "%s", msg);
+    return code;
+}
+
+struct albd_ctx {
+    int device;
+    int ncu;
+    hipStream_t stream;
+    hipEvent_t ev[ALBD_NEVENTS];
+    albd_counters *hctr; // pinned
+};
+
+int albd_device_count(int *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        (void)hipGetLastError();
+        return albd_set_error_msg(ALBBA_ENODEV, "no HIP device");
+    }
+    *count = n;
+    return 0;
+}
+
+int albd_ctx_create(int device, albd_ctx **out)
+{
+    int n = 0;
+    if (albd_device_count(&n) || device < 0 || device >= n)
+        return albd_set_error_msg(ALBBA_ENODEV, "requested HIP device is not present");
+    HIPCHK(hipSetDevice(device));
+    albd_ctx *c = new albd_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->ncu = prop.multiProcessorCount;
+    // default priority stream: no priority games
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 0; i < ALBD_NEVENTS; ++i) HIPCHK(hipEventCreate(&c->ev[i]));
+    HIPCHK(hipHostMalloc((void **)&c->hctr, sizeof(albd_counters), hipHostMallocDefault));
+    memset(c->hctr, 0, sizeof(albd_counters));
+    *out = c;
+    return 0;
+}
+
+void albd_ctx_destroy(albd_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < ALBD_NEVENTS; ++i) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
+    (void)hipHostFree(c->hctr);
+    delete c;
+}
+
+int albd_ctx_device(const albd_ctx *c) { return c->device; }
+int albd_num_cus(const albd_ctx *c) { return c->ncu; }
+hipStream_t albd_stream(albd_ctx *c) { return c->stream; }
+
+int albd_malloc(albd_ctx *c, size_t bytes, void **p)
+{
+    HIPCHK(hipSetDevice(c->device));
+    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        snprintf(g_err, sizeof g_err, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return ALBBA_EDEVMEM;
+    }
+    return 0;
+}
+
+void albd_free(albd_ctx *c, void *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(c->device);
+    (void)hipFree(p);
+}
+
+int albd_h2d(albd_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int albd_d2h(albd_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int albd_d2d(albd_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+int albd_sync(albd_ctx *c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+albd_counters *albd_host_counters(albd_ctx *c) { return c->hctr; }
+
+int albd_read_counters(albd_ctx *c, const albd_state *st)
+{
+    HIPCHK(hipMemcpyAsync(c->hctr, st->ctr, sizeof(albd_counters), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int albd_event_record(albd_ctx *c, int idx)
+{
+    HIPCHK(hipEventRecord(c->ev[idx], c->stream));
+    return 0;
+}
+
+int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms)
+{
+    HIPCHK(hipEventSynchronize(c->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fills
+
+static __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static __global__ __launch_bounds__(256) void k_fill_u64(uint64_t *p, uint64_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static unsigned fill_blocks(albd_ctx *c, size_t n)
+{
+    size_t b = (n + 255) / 256;
+    size_t cap = (size_t)c->ncu * 16;
+    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+int albd_fill_u32(albd_ctx *c, void *p, uint32_t v, size_t count)
+{
+    if (!count) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(c, count)), dim3(256), 0, c->stream,
+                       (uint32_t *)p, v, count);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_fill_u64(albd_ctx *c, void *p, uint64_t v, size_t count)
+{
+    if (!count) return 0;
+    hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(c, count)), dim3(256), 0, c->stream,
+                       (uint64_t *)p, v, count);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,209 @@
+// FP32 MLP 12 -> 32 -> 32 -> 4 (ReLU, ReLU, linear) of the learned direction
+// selector: forward, mean-squared-error loss with its exact gradient, Adam,
+// and the single-vector decision used inside the BFS level loop.
+//
+// Parameter layout (1604 floats): W1[32][12], b1[32], W2[32][32], b2[32],
+// W3[4][32], b3[4]; W[j][i] multiplies input i into unit j.
+//
+// The gradient is computed in two deterministic passes: one thread per
+// sample stores activations and back-propagated errors, then one thread per
+// parameter sums its sample contributions in sample order (no atomics), so a
+// run reproduces bit for bit and can be compared with a CPU restatement.
+#include "dev_common.h"
+#include "../../../include/albba.h"
+
+namespace {
+
+constexpr int NI = ALBBA_MLP_IN, H1 = ALBBA_MLP_H1, H2 = ALBBA_MLP_H2, NO = ALBBA_MLP_OUT;
+constexpr int OW1 = 0, OB1 = OW1 + H1 * NI, OW2 = OB1 + H1, OB2 = OW2 + H2 * H1,
+              OW3 = OB2 + H2, OB3 = OW3 + NO * H2, NP = OB3 + NO;
+static_assert(NP == ALBBA_MLP_NPARAM, "parameter count");
+
+__device__ __forceinline__ void load_params(const float *params, float *sp)
+{
+    for (int i = threadIdx.x; i < NP; i += blockDim.x) sp[i] = params[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void forward_one(const float *sp, const float *x, float *h1, float *h2,
+                                            float *y)
+{
+    for (int j = 0; j < H1; ++j) {
+        float a = sp[OB1 + j];
+        for (int i = 0; i < NI; ++i) a = fmaf(sp[OW1 + j * NI + i], x[i], a);
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < H2; ++j) {
+        float a = sp[OB2 + j];
+        for (int i = 0; i < H1; ++i) a = fmaf(sp[OW2 + j * H1 + i], h1[i], a);
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    for (int k = 0; k < NO; ++k) {
+        float a = sp[OB3 + k];
+        for (int i = 0; i < H2; ++i) a = fmaf(sp[OW3 + k * H2 + i], h2[i], a);
+        y[k] = a;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_forward(const float *params, const float *x, int32_t ns,
+                                                 float *out)
+{
+    __shared__ float sp[NP];
+    load_params(params, sp);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    float xi[NI], h1[H1], h2[H2], y[NO];
+    for (int i = 0; i < NI; ++i) xi[i] = x[(size_t)s * NI + i];
+    forward_one(sp, xi, h1, h2, y);
+    for (int k = 0; k < NO; ++k) out[(size_t)s * NO + k] = y[k];
+}
+
+// scratch layout per sample block: h1[ns][32] h2[ns][32] dy[ns][4] dh1[ns][32]
+// dh2[ns][32] lossv[ns]
+__global__ __launch_bounds__(256) void k_sample_pass(const float *params, const float *x,
+                                                     const float *t, int32_t ns, float *scratch)
+{
+    __shared__ float sp[NP];
+    load_params(params, sp);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    float *H1s = scratch, *H2s = H1s + (size_t)ns * H1, *DY = H2s + (size_t)ns * H2,
+          *DH1 = DY + (size_t)ns * NO, *DH2 = DH1 + (size_t)ns * H1, *LV = DH2 + (size_t)ns * H2;
+    float xi[NI], h1[H1], h2[H2], y[NO], dy[NO], dh2[H2];
+    for (int i = 0; i < NI; ++i) xi[i] = x[(size_t)s * NI + i];
+    forward_one(sp, xi, h1, h2, y);
+    const float scale = 2.0f / ((float)ns * (float)NO);
+    float l = 0.f;
+    for (int k = 0; k < NO; ++k) {
+        const float e = y[k] - t[(size_t)s * NO + k];
+        l = fmaf(e, e, l);
+        dy[k] = scale * e;
+        DY[(size_t)s * NO + k] = dy[k];
+    }
+    LV[s] = l;
+    for (int i = 0; i < H2; ++i) {
+        float a = 0.f;
+        for (int k = 0; k < NO; ++k) a = fmaf(sp[OW3 + k * H2 + i], dy[k], a);
+        dh2[i] = h2[i] > 0.f ? a : 0.f;
+        DH2[(size_t)s * H2 + i] = dh2[i];
+        H2s[(size_t)s * H2 + i] = h2[i];
+    }
+    for (int i = 0; i < H1; ++i) {
+        float a = 0.f;
+        for (int j = 0; j < H2; ++j) a = fmaf(sp[OW2 + j * H1 + i], dh2[j], a);
+        DH1[(size_t)s * H1 + i] = h1[i] > 0.f ? a : 0.f;
+        H1s[(size_t)s * H1 + i] = h1[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_param_pass(const float *x, int32_t ns,
+                                                    const float *scratch, float *grad, float *loss)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const float *H1s = scratch, *H2s = H1s + (size_t)ns * H1, *DY = H2s + (size_t)ns * H2,
+                *DH1 = DY + (size_t)ns * NO, *DH2 = DH1 + (size_t)ns * H1, *LV = DH2 + (size_t)ns * H2;
+    if (p == NP) {
+        float l = 0.f;
+        for (int s = 0; s < ns; ++s) l += LV[s];
+        *loss = l / ((float)ns * (float)NO);
+        return;
+    }
+    if (p > NP) return;
+    float g = 0.f;
+    if (p < OB1) {
+        const int j = (p - OW1) / NI, i = (p - OW1) % NI;
+        for (int s = 0; s < ns; ++s) g = fmaf(DH1[(size_t)s * H1 + j], x[(size_t)s * NI + i], g);
+    } else if (p < OW2) {
+        const int j = p - OB1;
+        for (int s = 0; s < ns; ++s) g += DH1[(size_t)s * H1 + j];
+    } else if (p < OB2) {
+        const int j = (p - OW2) / H1, i = (p - OW2) % H1;
+        for (int s = 0; s < ns; ++s) g = fmaf(DH2[(size_t)s * H2 + j], H1s[(size_t)s * H1 + i], g);
+    } else if (p < OW3) {
+        const int j = p - OB2;
+        for (int s = 0; s < ns; ++s) g += DH2[(size_t)s * H2 + j];
+    } else if (p < OB3) {
+        const int k = (p - OW3) / H2, i = (p - OW3) % H2;
+        for (int s = 0; s < ns; ++s) g = fmaf(DY[(size_t)s * NO + k], H2s[(size_t)s * H2 + i], g);
+    } else {
+        const int k = p - OB3;
+        for (int s = 0; s < ns; ++s) g += DY[(size_t)s * NO + k];
+    }
+    grad[p] = g;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float *params, const float *grad, float *m, float *v,
+                                              int32_t np, float lr, float b1, float b2, float eps,
+                                              float c1, float c2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const float g = grad[p];
+    const float mm = b1 * m[p] + (1.f - b1) * g;
+    const float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+__global__ void k_decide(const float *params, const float *feat, int32_t valid_mask, int32_t *choice)
+{
+    __shared__ float sp[NP];
+    load_params(params, sp);
+    if (threadIdx.x != 0) return;
+    float xi[NI], h1[H1], h2[H2], y[NO];
+    for (int i = 0; i < NI; ++i) xi[i] = feat[i];
+    forward_one(sp, xi, h1, h2, y);
+    int best = -1;
+    float bv = 0.f;
+    for (int k = 0; k < NO; ++k) {
+        if (!((valid_mask >> k) & 1)) continue;
+        if (!(y[k] == y[k])) continue; // NaN: not a candidate
+        if (best < 0 || y[k] < bv) { best = k; bv = y[k]; }
+    }
+    *choice = best;
+}
+
+} // namespace
+
+extern "C" {
+
+int albd_mlp_forward(albd_ctx *c, const float *params, const float *x, int32_t ns, float *out)
+{
+    if (ns <= 0) return 0;
+    hipLaunchKernelGGL(k_forward, dim3((ns + 255) / 256), dim3(256), 0, albd_stream(c), params, x, ns, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_mlp_loss_grad(albd_ctx *c, const float *params, const float *x, const float *t, int32_t ns,
+                       float *scratch, float *grad, float *loss)
+{
+    if (ns <= 0) return albd_set_error_msg(ALBBA_EINVAL, "mlp: no samples");
+    hipStream_t s = albd_stream(c);
+    hipLaunchKernelGGL(k_sample_pass, dim3((ns + 255) / 256), dim3(256), 0, s, params, x, t, ns, scratch);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_param_pass, dim3((NP + 1 + 255) / 256), dim3(256), 0, s, x, ns, scratch, grad, loss);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_mlp_adam(albd_ctx *c, float *params, const float *grad, float *m, float *v, int32_t np,
+                  float lr, float b1, float b2, float eps, int32_t step)
+{
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, albd_stream(c), params, grad, m, v,
+                       np, lr, b1, b2, eps, c1, c2);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_mlp_decide(albd_ctx *c, const float *params, const float *feat, int32_t valid_mask,
+                    int32_t *choice)
+{
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, albd_stream(c), params, feat, valid_mask, choice);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,536 @@
+/*
+ * BFS driver: device graph upload, the level loop with its direction switch,
+ * and the reference-compatible run_custom_bfs_test.
+ *
+ * Level loop (bfs/src/bfs_SELLCSv2.c:294-765 restated for the GPU): every
+ * level either pulls (SpMV over unvisited slices) or pushes (SpMSpV from the
+ * frontier queue); the frontier lives in a position-space bitmap (always)
+ * and a queue (after a push, or rebuilt from the bitmap when a push follows
+ * a pull).  The host reads one small counter block per level; all per-vertex
+ * work stays on the device.  The reference's switch (bfs_SELLCSv2.c:282-290,
+ * 302, 344, 356) pulls when the frontier holds more than (int)(0.04 n)
+ * vertices; it is the default policy, the fallback of the learned selector
+ * and the yardstick for its lost time.
+ */
+#include "internal.h"
+#include "../albd.h"
+#include "selector_int.h"
+
+#include <math.h>
+
+int sigma_window = 16384;
+int perf_analysis = 0;
+int verify = 0;
+
+struct albba_ctx {
+    albd_ctx *d;
+    int device;
+};
+
+struct albba_graph {
+    albba_ctx *ctx;
+    albd_graph g;
+    albd_state st;
+    int32_t n;
+    int64_t nnz;
+    int32_t *pos_of;     /* host: vertex id -> position */
+    int64_t device_bytes;
+    void *scratch_visited; /* BEST-policy snapshots */
+    void *scratch_ctr;
+    void *feat_dev;        /* [12] features of the level being decided */
+    void *choice_dev;      /* int32 choice written by the select kernel */
+};
+
+/* ---- context ---- */
+
+int albba_device_count(int *count)
+{
+    if (!count) return albba_fail(ALBBA_EINVAL, "device_count: null");
+    int rc = albd_device_count(count);
+    if (rc) return albba_fail(rc, "%s", albd_error());
+    return ALBBA_OK;
+}
+
+int albba_ctx_create(int device, albba_ctx **out)
+{
+    if (!out) return albba_fail(ALBBA_EINVAL, "ctx_create: null");
+    albba_ctx *c = (albba_ctx *)albba_calloc(1, sizeof(albba_ctx));
+    if (!c) return albba_fail(ALBBA_ENOMEM, "ctx_create");
+    int rc = albd_ctx_create(device, &c->d);
+    if (rc) {
+        free(c);
+        return albba_fail(rc, "%s", albd_error());
+    }
+    c->device = device;
+    *out = c;
+    return ALBBA_OK;
+}
+
+void albba_ctx_destroy(albba_ctx *c)
+{
+    if (!c) return;
+    albd_ctx_destroy(c->d);
+    free(c);
+}
+
+int albba_ctx_synchronize(albba_ctx *c)
+{
+    if (!c) return albba_fail(ALBBA_EINVAL, "synchronize: null");
+    int rc = albd_sync(c->d);
+    return rc ? albba_fail(rc, "%s", albd_error()) : ALBBA_OK;
+}
+
+albd_ctx *albba_ctx_dev(albba_ctx *c) { return c ? c->d : NULL; }
+
+/* ---- upload ---- */
+
+static void *dalloc(albba_graph *G, size_t bytes, int *rc)
+{
+    void *p = NULL;
+    if (*rc) return NULL;
+    *rc = albd_malloc(G->ctx->d, bytes, &p);
+    if (!*rc) G->device_bytes += (int64_t)bytes;
+    return p;
+}
+
+static int upload_padded_i64(albba_ctx *c, int64_t *dst, const int64_t *src, int64_t n_src,
+                             int64_t n_dst)
+{
+    int64_t *tmp = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)n_dst);
+    if (!tmp) return ALBBA_ENOMEM;
+    memcpy(tmp, src, sizeof(int64_t) * (size_t)n_src);
+    for (int64_t i = n_src; i < n_dst; ++i) tmp[i] = src[n_src - 1];
+    int rc = albd_h2d(c->d, dst, tmp, sizeof(int64_t) * (size_t)n_dst);
+    free(tmp);
+    return rc;
+}
+
+static int upload_padded_i32(albba_ctx *c, int32_t *dst, const int32_t *src, int64_t n_src,
+                             int64_t n_dst, int32_t fill)
+{
+    int32_t *tmp = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n_dst);
+    if (!tmp) return ALBBA_ENOMEM;
+    memcpy(tmp, src, sizeof(int32_t) * (size_t)n_src);
+    for (int64_t i = n_src; i < n_dst; ++i) tmp[i] = fill;
+    int rc = albd_h2d(c->d, dst, tmp, sizeof(int32_t) * (size_t)n_dst);
+    free(tmp);
+    return rc;
+}
+
+void albba_graph_free(albba_graph *G)
+{
+    if (!G) return;
+    albd_ctx *d = G->ctx->d;
+    albd_free(d, (void *)G->g.slice_ptr);
+    albd_free(d, (void *)G->g.slice_width);
+    albd_free(d, (void *)G->g.vop_ptr);
+    albd_free(d, (void *)G->g.vop_len);
+    albd_free(d, (void *)G->g.colidx);
+    albd_free(d, (void *)G->g.row_order);
+    albd_free(d, (void *)G->g.row_len);
+    albd_free(d, (void *)G->g.push_ptr);
+    albd_free(d, (void *)G->g.push_col);
+    albd_free(d, (void *)G->g.haslen);
+    albd_free(d, G->st.visited);
+    albd_free(d, G->st.front);
+    albd_free(d, G->st.next);
+    albd_free(d, G->st.summary);
+    albd_free(d, G->st.queue);
+    albd_free(d, G->st.queue_next);
+    albd_free(d, G->st.prefix);
+    albd_free(d, G->st.block_sums);
+    albd_free(d, G->st.pmin);
+    albd_free(d, G->st.level);
+    albd_free(d, G->st.parent);
+    albd_free(d, G->st.active);
+    albd_free(d, G->st.active_next);
+    albd_free(d, G->st.ctr);
+    albd_free(d, G->scratch_visited);
+    albd_free(d, G->scratch_ctr);
+    albd_free(d, G->feat_dev);
+    albd_free(d, G->choice_dev);
+    free(G->pos_of);
+    free(G);
+}
+
+int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
+{
+    if (!ctx || !s || !out) return albba_fail(ALBBA_EINVAL, "graph_upload: null argument");
+    albba_graph *G = (albba_graph *)albba_calloc(1, sizeof(albba_graph));
+    if (!G) return albba_fail(ALBBA_ENOMEM, "graph_upload");
+    G->ctx = ctx;
+    G->n = s->n;
+    G->nnz = s->nnz;
+    const int64_t npos_raw = (int64_t)s->nslices * s->C;
+    const int64_t nwords = (npos_raw + 63) / 64;
+    const int64_t npos = nwords * 64;
+    const int64_t ns_alloc = (npos + s->C - 1) / s->C;
+    albd_graph *g = &G->g;
+    g->n = s->n;
+    g->C = s->C;
+    g->pad = s->pad;
+    g->nslices = s->nslices;
+    g->nwords = (int32_t)nwords;
+    g->npos = npos;
+    g->nnz = s->nnz;
+    g->padded_nnz = s->padded_nnz;
+    g->nvops = s->nvops;
+    int rc = 0;
+    g->slice_ptr = (const int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(ns_alloc + 1), &rc);
+    g->slice_width = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)ns_alloc, &rc);
+    g->vop_ptr = (const int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(ns_alloc + 1), &rc);
+    g->vop_len = (const uint8_t *)dalloc(G, (size_t)(s->nvops > 0 ? s->nvops : 1), &rc);
+    g->colidx = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->padded_nnz > 0 ? s->padded_nnz : 1), &rc);
+    g->row_order = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    g->row_len = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    g->push_ptr = (const int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
+    g->push_col = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->nnz > 0 ? s->nnz : 1), &rc);
+    g->haslen = (const uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    albd_state *st = &G->st;
+    const int64_t nsum = (nwords + 63) / 64 + 1;
+    st->visited = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->front = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->next = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->summary = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nsum, &rc);
+    st->queue = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->queue_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->prefix = (int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
+    st->block_sums = (int64_t *)dalloc(G, sizeof(int64_t) * (ALBD_SCAN_BLOCKS + 1), &rc);
+    st->pmin = (uint32_t *)dalloc(G, sizeof(uint32_t) * (size_t)npos, &rc);
+    st->level = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
+    st->parent = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
+    st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
+    st->active_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
+    st->ctr = (albd_counters *)dalloc(G, sizeof(albd_counters), &rc);
+    G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
+    G->feat_dev = dalloc(G, sizeof(float) * 16, &rc);
+    G->choice_dev = dalloc(G, sizeof(int32_t) * 4, &rc);
+    if (rc) {
+        int code = rc;
+        const char *msg = albd_error();
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", msg);
+        albba_graph_free(G);
+        return albba_fail(code, "graph_upload: %s", buf);
+    }
+    /* host copies, padded to the device shapes */
+    if ((rc = upload_padded_i64(ctx, (int64_t *)g->slice_ptr, s->slice_pointers, s->nslices + 1, ns_alloc + 1)) ||
+        (rc = upload_padded_i32(ctx, (int32_t *)g->slice_width, s->slice_widths, s->nslices, ns_alloc, 0)) ||
+        (rc = upload_padded_i64(ctx, (int64_t *)g->vop_ptr, s->vop_pointers, s->nslices + 1, ns_alloc + 1)) ||
+        (rc = albd_h2d(ctx->d, (void *)g->vop_len, s->vop_lengths, (size_t)s->nvops)) ||
+        (rc = albd_h2d(ctx->d, (void *)g->colidx, s->column_indices, sizeof(int32_t) * (size_t)s->padded_nnz)) ||
+        (rc = upload_padded_i32(ctx, (int32_t *)g->row_order, s->row_order, s->n, npos, -1)) ||
+        (rc = upload_padded_i32(ctx, (int32_t *)g->row_len, s->row_lengths, npos_raw, npos, 0)) ||
+        (rc = upload_padded_i64(ctx, (int64_t *)g->push_ptr, s->push_ptr, npos_raw + 1, npos + 1)) ||
+        (rc = albd_h2d(ctx->d, (void *)g->push_col, s->push_col, sizeof(int32_t) * (size_t)s->nnz)) ||
+        (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
+        (rc = albd_sync(ctx->d))) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "host staging buffer" : albd_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_upload: %s", buf);
+    }
+    G->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)s->n);
+    if (!G->pos_of) { albba_graph_free(G); return albba_fail(ALBBA_ENOMEM, "graph_upload: pos_of"); }
+    memcpy(G->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n);
+    *out = G;
+    return ALBBA_OK;
+}
+
+int64_t albba_graph_device_bytes(const albba_graph *G) { return G ? G->device_bytes : 0; }
+
+/* ---- level loop ---- */
+
+void albba_bfs_opts_default(albba_bfs_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->switch_policy = ALBBA_SWITCH_HEURISTIC;
+    o->fixed_choice = ALBBA_CHOICE_PULL_SB;
+    o->timing_reps = 3;
+}
+
+enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4 };
+int albd_level_begin_flags(albd_ctx *c, const albd_graph *g, const albd_state *st, int flags);
+
+typedef struct loop_state {
+    int have_queue;       /* st.queue holds the frontier                */
+    int prev_push;        /* last level pushed (queue_next pending)     */
+    int list_valid;       /* st.active is a superset of unvisited words */
+    int list_pending;     /* last level built a list (length pending)   */
+    int64_t list_len;     /* host copy of the st.active length          */
+} loop_state;
+
+typedef struct level_sums {
+    int64_t next_count, next_edges, next_words, scanned, qnext_len, active_next;
+} level_sums;
+
+static void sum_counters(const albd_counters *h, level_sums *o)
+{
+    memset(o, 0, sizeof(*o));
+    for (int i = 0; i < ALBD_NSLOT; ++i) {
+        o->next_count += (int64_t)h->slot[i].next_count;
+        o->next_edges += (int64_t)h->slot[i].next_edges;
+        o->next_words += (int64_t)h->slot[i].next_words;
+        o->scanned += (int64_t)h->slot[i].scanned;
+    }
+    o->qnext_len = (int64_t)h->qnext_len;
+    o->active_next = (int64_t)h->active_next;
+}
+
+int albba_heuristic_choice(int32_t n, int64_t frontier)
+{
+    /* bfs/src/bfs_SELLCSv2.c:282-284: int threshold = n * 0.04 (truncated) */
+    const int thr = (int)(n * 0.04);
+    return frontier > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
+}
+
+/* Launch the kernels of one level (no pointer swaps). */
+static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t level,
+                     int64_t frontier)
+{
+    albd_ctx *d = G->ctx->d;
+    int flags = 0;
+    if (ls->prev_push) flags |= LB_QUEUE_FROM_NEXT;
+    if (ls->list_pending) flags |= LB_ACTIVE_FROM_NEXT;
+    if (choice == ALBBA_CHOICE_PUSH && !ls->have_queue) flags |= LB_ZERO_QUEUE;
+    int rc = albd_level_begin_flags(d, &G->g, &G->st, flags);
+    if (rc) return rc;
+    if (choice == ALBBA_CHOICE_PUSH) {
+        if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
+        if ((rc = albd_push_scan(d, &G->g, &G->st, frontier))) return rc;
+        if ((rc = albd_push_expand(d, &G->g, &G->st))) return rc;
+        return albd_push_finalize(d, &G->g, &G->st, level);
+    }
+    int variant = ALBD_PULL_ALL;
+    if (choice == ALBBA_CHOICE_PULL_SB) variant = ALBD_PULL_LIST;
+    if (choice == ALBBA_CHOICE_PULL_CB) variant = ALBD_PULL_LIST_CB;
+    if (variant != ALBD_PULL_ALL &&
+        (rc = albd_build_active(d, &G->g, &G->st, ls->list_valid, ls->list_len)))
+        return rc;
+    if (variant == ALBD_PULL_LIST_CB && (rc = albd_build_summary(d, &G->g, &G->st))) return rc;
+    return albd_pull(d, &G->g, &G->st, variant, level, 0);
+}
+
+/* Pointer swaps and bookkeeping after a committed level. */
+static void commit_level(albba_graph *G, loop_state *ls, int choice, const level_sums *sums)
+{
+    uint64_t *t = G->st.front;
+    G->st.front = G->st.next;
+    G->st.next = t;
+    if (ls->list_pending) {
+        /* the list built one level earlier is now st.active (see below) */
+        ls->list_pending = 0;
+    }
+    if (choice == ALBBA_CHOICE_PUSH) {
+        int32_t *q = G->st.queue;
+        G->st.queue = G->st.queue_next;
+        G->st.queue_next = q;
+        ls->have_queue = 1;
+        ls->prev_push = 1;
+    } else {
+        ls->have_queue = 0;
+        ls->prev_push = 0;
+        if (choice != ALBBA_CHOICE_PULL) {
+            int32_t *a = G->st.active;
+            G->st.active = G->st.active_next;
+            G->st.active_next = a;
+            ls->list_valid = 1;
+            ls->list_pending = 1;
+            ls->list_len = sums->active_next;
+        }
+    }
+}
+
+static int choice_ok(const albba_graph *G, int c)
+{
+    if (c < 0 || c >= ALBBA_NUM_CHOICES) return 0;
+    if (c == ALBBA_CHOICE_PULL_CB && !albd_cb_supported(&G->g)) return 0;
+    return 1;
+}
+
+static int cmp_float(const void *a, const void *b)
+{
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
+              albba_bfs_stats *stats)
+{
+    if (!ctx || !G || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "bfs: bad context/graph");
+    if (root < 0 || root >= G->n) return albba_fail(ALBBA_EINVAL, "bfs: root %d out of range [0,%d)", root, G->n);
+    albba_bfs_opts opts;
+    if (opts_in) opts = *opts_in;
+    else albba_bfs_opts_default(&opts);
+    const int policy = opts.switch_policy;
+    if (policy < ALBBA_SWITCH_HEURISTIC || policy > ALBBA_SWITCH_BEST)
+        return albba_fail(ALBBA_EINVAL, "bfs: unknown switch policy %d", policy);
+    if (policy == ALBBA_SWITCH_FIXED && !choice_ok(G, opts.fixed_choice))
+        return albba_fail(ALBBA_EINVAL, "bfs: fixed choice %d not available", opts.fixed_choice);
+    if (policy == ALBBA_SWITCH_LEARNED && !opts.selector)
+        return albba_fail(ALBBA_EINVAL, "bfs: learned policy needs a selector");
+    albd_ctx *d = ctx->d;
+    const int reps = opts.timing_reps > 0 ? opts.timing_reps : 1;
+    const int32_t rp = G->pos_of[root];
+    /* root out-degree for the Graph500 edge count */
+    int64_t pp[2];
+    int rc = albd_d2h(d, pp, G->g.push_ptr + rp, sizeof pp);
+    if (rc) return albba_fail(rc, "bfs: %s", albd_error());
+
+    loop_state ls = {1, 0, 0, 0, 0};
+    int64_t frontier = 1, frontier_edges = pp[1] - pp[0], nzwords = 1;
+    int64_t reached = 1, reached_edges = pp[1] - pp[0];
+    int32_t level = 0;
+    int prev_choice = ALBBA_CHOICE_PUSH;
+    float feats[ALBBA_NUM_FEATURES];
+
+    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
+        return albba_fail(rc, "bfs: %s", albd_error());
+
+    while (frontier > 0) {
+        albba_features(G->n, G->nnz, G->g.nwords, frontier, frontier_edges, nzwords, reached,
+                       reached_edges, level, prev_choice, feats);
+        int choice = albba_heuristic_choice(G->n, frontier);
+        float choice_ms[ALBBA_NUM_CHOICES] = {0, 0, 0, 0};
+        if (policy == ALBBA_SWITCH_FIXED) {
+            choice = opts.fixed_choice;
+        } else if (policy == ALBBA_SWITCH_SCRIPT && opts.script && level < opts.script_len &&
+                   choice_ok(G, opts.script[level])) {
+            choice = opts.script[level];
+        } else if (policy == ALBBA_SWITCH_LEARNED) {
+            int c2 = -1;
+            rc = albba_selector_decide(opts.selector, d, feats, G->feat_dev, G->choice_dev, &c2);
+            if (rc) return albba_fail(rc, "bfs: selector: %s", albd_error());
+            if (choice_ok(G, c2)) choice = c2; /* else: heuristic fallback */
+        } else if (policy == ALBBA_SWITCH_BEST) {
+            /* time every choice on a snapshot of the state, commit the fastest */
+            if ((rc = albd_d2d(d, G->scratch_visited, G->st.visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
+                (rc = albd_d2d(d, G->scratch_ctr, G->st.ctr, sizeof(albd_counters))))
+                return albba_fail(rc, "bfs: %s", albd_error());
+            float best = INFINITY;
+            for (int c = 0; c < ALBBA_NUM_CHOICES; ++c) {
+                if (!choice_ok(G, c)) { choice_ms[c] = INFINITY; continue; }
+                float t[16];
+                int nr = reps < 16 ? reps : 16;
+                for (int r = 0; r < nr; ++r) {
+                    if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
+                        (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))) ||
+                        (rc = albd_event_record(d, 4)) || (rc = run_level(G, &ls, c, level, frontier)) ||
+                        (rc = albd_event_record(d, 5)) || (rc = albd_event_elapsed(d, 4, 5, &t[r])))
+                        return albba_fail(rc, "bfs: %s", albd_error());
+                }
+                qsort(t, (size_t)nr, sizeof(float), cmp_float);
+                choice_ms[c] = t[nr / 2];
+                if (choice_ms[c] < best) { best = choice_ms[c]; choice = c; }
+            }
+            if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
+                (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))))
+                return albba_fail(rc, "bfs: %s", albd_error());
+        }
+        if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
+        if ((rc = run_level(G, &ls, choice, level, frontier))) return albba_fail(rc, "bfs: %s", albd_error());
+        if (opts.time_levels && (rc = albd_event_record(d, 3))) return albba_fail(rc, "bfs: %s", albd_error());
+        if ((rc = albd_read_counters(d, &G->st))) return albba_fail(rc, "bfs: %s", albd_error());
+        level_sums sums;
+        sum_counters(albd_host_counters(d), &sums);
+        if (stats && level < stats->capacity) {
+            if (stats->frontier) stats->frontier[level] = (int32_t)frontier;
+            if (stats->choice) stats->choice[level] = choice;
+            if (stats->features) memcpy(stats->features + (size_t)level * ALBBA_NUM_FEATURES, feats, sizeof feats);
+            if (stats->choice_ms) memcpy(stats->choice_ms + (size_t)level * ALBBA_NUM_CHOICES, choice_ms, sizeof choice_ms);
+            if (stats->level_ms) {
+                float ms = 0;
+                if (opts.time_levels) albd_event_elapsed(d, 2, 3, &ms);
+                stats->level_ms[level] = ms;
+            }
+        }
+        commit_level(G, &ls, choice, &sums);
+        prev_choice = choice;
+        frontier = sums.next_count;
+        frontier_edges = sums.next_edges;
+        nzwords = sums.next_words;
+        reached += sums.next_count;
+        reached_edges += sums.next_edges;
+        ++level;
+    }
+    float ms = 0;
+    if ((rc = albd_event_record(d, 1)) || (rc = albd_event_elapsed(d, 0, 1, &ms)))
+        return albba_fail(rc, "bfs: %s", albd_error());
+    if (stats) {
+        stats->num_levels = level;
+        stats->reached = reached;
+        stats->edges_in_component = reached_edges;
+        stats->bfs_ms = ms;
+    }
+    return ALBBA_OK;
+}
+
+int albba_bfs_results(albba_ctx *ctx, albba_graph *G, int32_t *level, int32_t *parent)
+{
+    if (!ctx || !G) return albba_fail(ALBBA_EINVAL, "bfs_results: null");
+    int rc = 0;
+    if (level && (rc = albd_d2h(ctx->d, level, G->st.level, sizeof(int32_t) * (size_t)G->n)))
+        return albba_fail(rc, "bfs_results: %s", albd_error());
+    if (parent && (rc = albd_d2h(ctx->d, parent, G->st.parent, sizeof(int32_t) * (size_t)G->n)))
+        return albba_fail(rc, "bfs_results: %s", albd_error());
+    return ALBBA_OK;
+}
+
+/* ---- bfs/src/bfs.h:13 compatibility ---- */
+
+void run_custom_bfs_test(const SparseMatrixCSR *matrix, elem_t *bfs_x, int *colidx_x, int *nnzx,
+                         int *frontier, int *num_vex, double *conv_time, double *iter_times,
+                         double *bfs_time, int *num_iter, bool bit_flag, bool sort_flag)
+{
+    (void)bfs_x;
+    (void)colidx_x;
+    (void)bit_flag;
+    (void)sort_flag;
+    const int n = matrix->nrows;
+    albba_csr g;
+    g.n = n;
+    g.nnz = matrix->nnz;
+    g.symmetric = 0;
+    g.row_ptr = (int64_t *)albba_malloc(sizeof(int64_t) * ((size_t)n + 1));
+    g.col = matrix->column_indices;
+    if (!g.row_ptr) { fprintf(stderr, "run_custom_bfs_test: out of memory\n"); exit(1); }
+    for (int i = 0; i <= n; ++i) g.row_ptr[i] = matrix->row_pointers[i];
+    int sigma = sigma_window < 64 ? 64 : (sigma_window / 64) * 64;
+    double t0 = albba_now_ms();
+    albba_sellcs *s = NULL;
+    int rc = albba_sellcs_build(&g, 64, sigma, 32, &s);
+    *conv_time = albba_now_ms() - t0;
+    free(g.row_ptr);
+    albba_ctx *ctx = NULL;
+    albba_graph *G = NULL;
+    if (rc || (rc = albba_ctx_create(0, &ctx)) || (rc = albba_graph_upload(ctx, s, &G))) {
+        fprintf(stderr, "run_custom_bfs_test: %s\n", albba_last_error());
+        exit(1);
+    }
+    albba_sellcs_free(s);
+    int cap = n > 0 ? n : 1;
+    float *lms = (float *)albba_calloc((size_t)cap, sizeof(float));
+    albba_bfs_stats st;
+    memset(&st, 0, sizeof st);
+    st.capacity = cap;
+    st.frontier = frontier;
+    st.level_ms = lms;
+    albba_bfs_opts o;
+    albba_bfs_opts_default(&o);
+    o.time_levels = 1;
+    if ((rc = albba_bfs(ctx, G, 0, &o, &st))) {
+        fprintf(stderr, "run_custom_bfs_test: %s\n", albba_last_error());
+        exit(1);
+    }
+    *num_iter = st.num_levels;
+    *num_vex = (int)st.reached;
+    *bfs_time = st.bfs_ms;
+    if (iter_times)
+        for (int i = 0; i < st.num_levels && i < cap; ++i) iter_times[i] = lms[i];
+    *nnzx = 0;
+    free(lms);
+    albba_graph_free(G);
+    albba_ctx_destroy(ctx);
+}
