@@ -91,7 +91,8 @@ class _Opts(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("num_levels", C.c_int32), ("capacity", C.c_int32), ("reached", C.c_int64),
                 ("edges_in_component", C.c_int64), ("bfs_ms", C.c_double), ("frontier", I32P),
-                ("choice", I32P), ("level_ms", F32P), ("features", F32P), ("choice_ms", F32P)]
+                ("choice", I32P), ("level_ms", F32P), ("features", F32P), ("choice_ms", F32P),
+                ("entries", I64P), ("words", I64P), ("bypassed", I64P), ("kernel_ms", F32P)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -328,14 +329,28 @@ def device_count() -> int:
 
 
 class Context:
+    """One HIP device + stream.  Objects created on it (device graphs,
+    selectors) are closed before the context itself."""
+
     def __init__(self, device: int = 0):
         self._p = C.c_void_p()
+        self._children = []
         _check(lib().albba_ctx_create(device, C.byref(self._p)))
+
+    def _adopt(self, obj):
+        import weakref
+
+        self._children.append(weakref.ref(obj))
 
     def synchronize(self):
         _check(lib().albba_ctx_synchronize(self._p))
 
     def close(self):
+        for ref in getattr(self, "_children", []):
+            obj = ref()
+            if obj is not None:
+                obj.close()
+        self._children = []
         if getattr(self, "_p", None) and self._p.value and _lib is not None:
             _lib.albba_ctx_destroy(self._p)
             self._p = C.c_void_p()
@@ -351,6 +366,7 @@ class DeviceGraph:
         self.nnz = int(sell.raw.nnz)
         self._p = C.c_void_p()
         _check(lib().albba_graph_upload(ctx._p, sell._p, C.byref(self._p)))
+        ctx._adopt(self)
 
     @property
     def device_bytes(self) -> int:
@@ -387,8 +403,14 @@ class DeviceGraph:
         lm = np.zeros(cap, np.float32)
         ft = np.zeros(cap * NUM_FEATURES, np.float32)
         cm = np.zeros(cap * 4, np.float32)
+        en = np.zeros(cap, np.int64)
+        wd = np.zeros(cap, np.int64)
+        by = np.zeros(cap, np.int64)
+        km = np.zeros(cap, np.float32)
         st = _Stats(0, cap, 0, 0, 0.0, fr.ctypes.data_as(I32P), ch.ctypes.data_as(I32P),
-                    lm.ctypes.data_as(F32P), ft.ctypes.data_as(F32P), cm.ctypes.data_as(F32P))
+                    lm.ctypes.data_as(F32P), ft.ctypes.data_as(F32P), cm.ctypes.data_as(F32P),
+                    en.ctypes.data_as(I64P), wd.ctypes.data_as(I64P), by.ctypes.data_as(I64P),
+                    km.ctypes.data_as(F32P))
         _check(lib().albba_bfs(self.ctx._p, self._p, int(root), C.byref(o), C.byref(st)))
         L = min(st.num_levels, cap)
         out = {
@@ -396,7 +418,8 @@ class DeviceGraph:
             "edges_in_component": int(st.edges_in_component), "bfs_ms": float(st.bfs_ms),
             "frontier": fr[:L].copy(), "choice": ch[:L].copy(), "level_ms": lm[:L].copy(),
             "features": ft[:L * NUM_FEATURES].reshape(L, NUM_FEATURES).copy(),
-            "choice_ms": cm[:L * 4].reshape(L, 4).copy(),
+            "choice_ms": cm[:L * 4].reshape(L, 4).copy(), "entries": en[:L].copy(),
+            "words": wd[:L].copy(), "bypassed": by[:L].copy(), "kernel_ms": km[:L].copy(),
         }
         if results:
             out["level"], out["parent"] = self.results()
@@ -417,6 +440,7 @@ class Selector:
         self.ctx = ctx
         self._p = C.c_void_p()
         _check(lib().albba_selector_create(ctx._p, seed, C.byref(self._p)))
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "_p", None) and self._p.value and _lib is not None:

@@ -26,7 +26,9 @@ typedef struct albd_slot {
     unsigned long long next_edges;   /* sum of out-degree of those vertices   */
     unsigned long long next_words;   /* nonzero next-frontier words           */
     unsigned long long scanned;      /* pull: column entries gathered         */
-    unsigned long long pad_[4];      /* one slot per 64 bytes                 */
+    unsigned long long words;        /* pull: position words scanned          */
+    unsigned long long bypassed;     /* pull: words skipped, all visited      */
+    unsigned long long pad_[2];      /* one slot per 64 bytes                 */
 } albd_slot;
 
 typedef struct albd_counters {

@@ -39,13 +39,17 @@ struct PullArgs {
 
 __device__ __forceinline__ void slot_add(albd_counters *ctr, unsigned long long cnt,
                                          unsigned long long edges, unsigned long long words,
-                                         unsigned long long scanned)
+                                         unsigned long long scanned,
+                                         unsigned long long words_scanned = 0,
+                                         unsigned long long bypassed = 0)
 {
     albd_slot *s = &ctr->slot[blockIdx.x % ALBD_NSLOT];
     if (cnt) atomicAdd(&s->next_count, cnt);
     if (edges) atomicAdd(&s->next_edges, edges);
     if (words) atomicAdd(&s->next_words, words);
     if (scanned) atomicAdd(&s->scanned, scanned);
+    if (words_scanned) atomicAdd(&s->words, words_scanned);
+    if (bypassed) atomicAdd(&s->bypassed, bypassed);
 }
 
 // ---------------------------------------------------------------------------
@@ -79,13 +83,17 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
     const int grp = (lane / SUB) * SUB;
     const int sl = lane % SUB;
 
-    unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0;
+    unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0, wscan = 0, wbyp = 0;
 
     for (int64_t it = lo + (int64_t)bx * kWavesPerBlock + wib; it < hi; it += stride) {
         const int32_t w = (VAR == ALBD_PULL_ALL) ? (int32_t)it : a.items[it];
         const uint64_t vis = a.st.visited[w];
         uint64_t unres = g.haslen[w] & ~vis;
-        if (unres == 0) continue;
+        if (unres == 0) {
+            ++wbyp; // slice bypass: every lane with a row is visited
+            continue;
+        }
+        ++wscan;
         const uint64_t start = unres;
         const int64_t p = (int64_t)w * 64 + lane;
         int s, l;
@@ -167,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
         }
     }
     edges = wave_sum(edges);
-    if (lane == 0) slot_add(a.st.ctr, cnt, edges, nzw, scanned);
+    if (lane == 0) slot_add(a.st.ctr, cnt, edges, nzw, scanned, wscan, wbyp);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,7 +488,8 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
     for (int64_t ch = c_lo; ch < c_hi; ++ch) {
         e0 = ch << 6;
         const int64_t e = e0 + lane;
-        // queue entries have degree >= 1, so the chunk spans v0 .. v0+63
+        // v0 holds edge e0 and queue entries have degree >= 1, so the chunk's
+        // edges belong to vertices v0 .. v0+63
         const int64_t Pk = (v0 + 1 + lane <= qlen) ? st.prefix[v0 + 1 + lane] : LLONG_MAX;
         const int64_t P0 = st.prefix[v0];
         int t = 0;
@@ -502,8 +511,10 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
                          1ull << (v & 63));
             }
         }
-        // vertex of the chunk's last edge starts the next chunk's search
-        v0 = __shfl(ui, 63, 64);
+        // the next chunk starts at edge e0 + 64; its vertex is v0 plus the
+        // number of prefix entries v0+1 .. v0+64 at or below that edge (the
+        // lanes hold exactly those 64 entries, so this count may reach 64)
+        v0 += __popcll(__ballot(Pk <= e0 + 64));
     }
 }
 

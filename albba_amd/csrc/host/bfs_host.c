@@ -263,7 +263,8 @@ typedef struct loop_state {
 } loop_state;
 
 typedef struct level_sums {
-    int64_t next_count, next_edges, next_words, scanned, qnext_len, active_next;
+    int64_t next_count, next_edges, next_words, scanned, words, bypassed;
+    int64_t qnext_len, active_next, queue_len, push_edges;
 } level_sums;
 
 static void sum_counters(const albd_counters *h, level_sums *o)
@@ -274,9 +275,13 @@ static void sum_counters(const albd_counters *h, level_sums *o)
         o->next_edges += (int64_t)h->slot[i].next_edges;
         o->next_words += (int64_t)h->slot[i].next_words;
         o->scanned += (int64_t)h->slot[i].scanned;
+        o->words += (int64_t)h->slot[i].words;
+        o->bypassed += (int64_t)h->slot[i].bypassed;
     }
     o->qnext_len = (int64_t)h->qnext_len;
     o->active_next = (int64_t)h->active_next;
+    o->queue_len = (int64_t)h->queue_len;
+    o->push_edges = (int64_t)h->push_edges;
 }
 
 int albba_heuristic_choice(int32_t n, int64_t frontier)
@@ -286,9 +291,10 @@ int albba_heuristic_choice(int32_t n, int64_t frontier)
     return frontier > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
 }
 
-/* Launch the kernels of one level (no pointer swaps). */
+/* Launch the kernels of one level (no pointer swaps).  With `mark` set, HIP
+ * events 6 and 7 bracket the main kernel (pull, or push expansion). */
 static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t level,
-                     int64_t frontier)
+                     int64_t frontier, int mark)
 {
     albd_ctx *d = G->ctx->d;
     int flags = 0;
@@ -300,7 +306,9 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
     if (choice == ALBBA_CHOICE_PUSH) {
         if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
         if ((rc = albd_push_scan(d, &G->g, &G->st, frontier))) return rc;
+        if (mark && (rc = albd_event_record(d, 6))) return rc;
         if ((rc = albd_push_expand(d, &G->g, &G->st))) return rc;
+        if (mark && (rc = albd_event_record(d, 7))) return rc;
         return albd_push_finalize(d, &G->g, &G->st, level);
     }
     int variant = ALBD_PULL_ALL;
@@ -310,7 +318,9 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
         (rc = albd_build_active(d, &G->g, &G->st, ls->list_valid, ls->list_len)))
         return rc;
     if (variant == ALBD_PULL_LIST_CB && (rc = albd_build_summary(d, &G->g, &G->st))) return rc;
-    return albd_pull(d, &G->g, &G->st, variant, level, 0);
+    if (mark && (rc = albd_event_record(d, 6))) return rc;
+    if ((rc = albd_pull(d, &G->g, &G->st, variant, level, 0))) return rc;
+    return mark ? albd_event_record(d, 7) : 0;
 }
 
 /* Pointer swaps and bookkeeping after a committed level. */
@@ -415,9 +425,12 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
                 float t[16];
                 int nr = reps < 16 ? reps : 16;
                 for (int r = 0; r < nr; ++r) {
+                    /* a choice 4x slower than the best so far cannot win:
+                     * one trial is enough to rank it */
+                    if (r == 1 && t[0] > 4.0f * best) { nr = 1; break; }
                     if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
                         (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))) ||
-                        (rc = albd_event_record(d, 4)) || (rc = run_level(G, &ls, c, level, frontier)) ||
+                        (rc = albd_event_record(d, 4)) || (rc = run_level(G, &ls, c, level, frontier, 0)) ||
                         (rc = albd_event_record(d, 5)) || (rc = albd_event_elapsed(d, 4, 5, &t[r])))
                         return albba_fail(rc, "bfs: %s", albd_error());
                 }
@@ -430,7 +443,8 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
                 return albba_fail(rc, "bfs: %s", albd_error());
         }
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
-        if ((rc = run_level(G, &ls, choice, level, frontier))) return albba_fail(rc, "bfs: %s", albd_error());
+        if ((rc = run_level(G, &ls, choice, level, frontier, opts.time_levels)))
+            return albba_fail(rc, "bfs: %s", albd_error());
         if (opts.time_levels && (rc = albd_event_record(d, 3))) return albba_fail(rc, "bfs: %s", albd_error());
         if ((rc = albd_read_counters(d, &G->st))) return albba_fail(rc, "bfs: %s", albd_error());
         level_sums sums;
@@ -445,6 +459,15 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
                 if (opts.time_levels) albd_event_elapsed(d, 2, 3, &ms);
                 stats->level_ms[level] = ms;
             }
+            if (stats->kernel_ms) {
+                float ms = 0;
+                if (opts.time_levels) albd_event_elapsed(d, 6, 7, &ms);
+                stats->kernel_ms[level] = ms;
+            }
+            const int push = choice == ALBBA_CHOICE_PUSH;
+            if (stats->entries) stats->entries[level] = push ? sums.push_edges : sums.scanned;
+            if (stats->words) stats->words[level] = push ? sums.queue_len : sums.words;
+            if (stats->bypassed) stats->bypassed[level] = push ? 0 : sums.bypassed;
         }
         commit_level(G, &ls, choice, &sums);
         prev_choice = choice;

@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: harmonic-mean GTEPS over 64 roots on RMAT scale 24 (edge factor
+16, a=0.57 b=0.19 c=0.19), learned vs heuristic direction switch, levels and
+parents bit-exact between the two.
+
+A step is one BFS from each of the 64 roots (one batch).  GTEPS follows the
+reference (bfs/src/report_results.c:36): stored directed entries / BFS time.
+The graph is resident in HBM before timing starts; the timed region covers
+the BFS level loops only.  Prints one JSON line on stdout; progress goes to
+stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "harmonic-mean GTEPS, 64 roots, RMAT s24; learned vs heuristic switch, bit-exact"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+SELECTOR = os.path.join(ROOT, "albba_amd", "selector.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pull_bytes(entries, words, n, n_new):
+    # 4 B per gathered column entry, 8 B slice metadata + 2 x (C/8) B visited
+    # read/write per scanned 64-lane word, one frontier-bitmap pass, 8 B of
+    # level+parent per discovered vertex
+    return 4 * entries + 8 * words + 16 * words + n / 8 + 8 * n_new
+
+
+def push_bytes(queue, edges, n, n_new):
+    # frontier list + row pointer pairs + adjacency + outputs + visited pass
+    return 4 * queue + 8 * queue + 4 * edges + 8 * n_new + n / 8
+
+
+def run_policy(dg, roots, policy, sel, steps, nnz, n, prof):
+    """Time `steps` passes over all roots; returns per-root mean ms, step ms."""
+    import albba_amd as A
+
+    per_root = np.zeros(len(roots))
+    step_ms = []
+    for s in range(steps):
+        t0 = time.perf_counter()
+        for i, r in enumerate(roots):
+            o = dg.bfs(int(r), policy=policy, selector=sel, time_levels=True, results=False)
+            per_root[i] += o["bfs_ms"] / steps
+            L = o["num_levels"]
+            for lv in range(L):
+                n_new = o["frontier"][lv + 1] if lv + 1 < L else 0
+                kind = "push" if o["choice"][lv] == A.CHOICE_PUSH else "pull"
+                b = (push_bytes(o["words"][lv], o["entries"][lv], n, n_new) if kind == "push"
+                     else pull_bytes(o["entries"][lv], o["words"][lv], n, n_new))
+                p = prof.setdefault(kind, {"ms": 0.0, "bytes": 0.0, "launches": 0})
+                p["ms"] += float(o["kernel_ms"][lv])
+                p["bytes"] += float(b)
+                p["launches"] += 1
+        dg.ctx.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"  step {s}: {step_ms[-1]:.1f} ms")
+    return per_root, float(np.mean(step_ms))
+
+
+def cpu_baseline(g, roots, gpu_levels):
+    """The oracle's C + OpenMP restatement of ALBBA (reference C=256,
+    sigma=16384, tasks = threads) on a bounded sample of the same roots."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rp, col = g.row_ptr, g.col
+    t = time.time()
+    s = O.build_sell(rp, col, 256, 16384)
+    log(f"cpu baseline: oracle SELL build {time.time() - t:.1f}s")
+    times, parity = [], True
+    budget_s, spent = 20.0, 0.0
+    for i, r in enumerate(roots):
+        t0 = time.perf_counter()
+        res = O.albba_bfs_on(s, g.n, int(r), tasks=threads)
+        dt = time.perf_counter() - t0
+        times.append(dt * 1e3)
+        spent += dt
+        if i in gpu_levels:
+            parity &= bool(np.array_equal(res["level"], gpu_levels[i][0]) and
+                           np.array_equal(res["parent"], gpu_levels[i][1]))
+        log(f"cpu baseline: root {r} {dt * 1e3:.0f} ms")
+        if spent > budget_s or i >= 7:
+            break
+    O.free_sell(s)
+    import albba_amd as A
+
+    gt = [A.gteps(g.nnz, ms) for ms in times]
+    return {"value": round(A.harmonic_mean(gt), 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"ALBBA level loop restated in C+OpenMP (oracle), {len(times)} of the same "
+                      f"roots, same RMAT s24 graph, SELL C=256 sigma=16384, {threads} tasks; "
+                      f"harmonic mean", "levels_match_gpu": parity}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a single BFS would shard as a 1D slice partition, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import albba_amd as A
+
+    t = time.time()
+    g = A.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, perm_seed=2)
+    log(f"graph RMAT s{args.scale}: n={g.n} nnz={g.nnz} ({time.time() - t:.1f}s)")
+    roots = g.pick_roots(args.roots, 7)
+    ctx = A.Context(0)
+    t = time.time()
+    sell = g.sellcs(64, 4096, 1)
+    dg = A.DeviceGraph(ctx, sell)
+    del sell
+    log(f"SELL C=64 sigma=4096 pad=1 built and uploaded ({time.time() - t:.1f}s, "
+        f"{dg.device_bytes / 2**30:.2f} GiB on device)")
+
+    sel = None
+    if os.path.exists(SELECTOR):
+        sel = A.Selector(ctx)
+        sel.load(SELECTOR)
+    policies = [("heuristic", A.SWITCH_HEURISTIC, None)]
+    if sel is not None:
+        policies.append(("learned", A.SWITCH_LEARNED, sel))
+
+    # warmup + bit-exactness: every policy must give identical level/parent
+    bit_exact = True
+    keep = {}
+    for w in range(max(args.warmup, 1)):
+        for i, r in enumerate(roots):
+            ref = None
+            for name, pol, s in policies:
+                o = dg.bfs(int(r), policy=pol, selector=s, results=(w == 0))
+                if w == 0:
+                    if ref is None:
+                        ref = (o["level"], o["parent"])
+                        if i < 8:
+                            keep[i] = ref
+                    else:
+                        bit_exact &= bool(np.array_equal(ref[0], o["level"]) and
+                                          np.array_equal(ref[1], o["parent"]))
+    log(f"warmup done, policies bit-exact: {bit_exact}")
+
+    results = {}
+    for name, pol, s in policies:
+        prof = {}
+        log(f"timing policy {name}")
+        per_root, step_ms = run_policy(dg, roots, pol, s, args.steps, g.nnz, g.n, prof)
+        gt = [A.gteps(g.nnz, ms) for ms in per_root]
+        results[name] = {"hm_gteps_events": A.harmonic_mean(gt), "step_ms": step_ms,
+                         "wall_gteps": len(roots) * g.nnz / (step_ms * 1e6), "prof": prof,
+                         "mean_bfs_ms": float(np.mean(per_root))}
+        log(f"  {name}: harmonic-mean GTEPS (events) {results[name]['hm_gteps_events']:.2f}, "
+            f"wall {results[name]['wall_gteps']:.2f}, step {step_ms:.1f} ms")
+
+    main_name = "learned" if "learned" in results else "heuristic"
+    R = results[main_name]
+    prof = R["prof"]
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    P = prof[dom]
+    achieved = P["bytes"] / (P["ms"] * 1e-3) if P["ms"] > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_pull" if dom == "pull" else "k_push_expand",
+                "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "avg_launch_ms": round(P["ms"] / max(P["launches"], 1), 5),
+                "avg_bytes_per_launch": round(P["bytes"] / max(P["launches"], 1)),
+                "launches": P["launches"]}
+    line = {
+        "metric": METRIC, "value": round(R["wall_gteps"], 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(R["step_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded RMAT (splitmix64 counter RNG, Fisher-Yates permuted ids, "
+                "symmetrized, deduplicated, no self loops)",
+        "config": {"workload": f"rmat_s{args.scale}_ef16_{len(roots)}roots",
+                   "scale": args.scale, "edgefactor": 16, "a": 0.57, "b": 0.19, "c": 0.19,
+                   "n": g.n, "nnz_stored": g.nnz, "roots": len(roots), "root_seed": 7,
+                   "layout": "SELL-C-sigma C=64 sigma=4096 pad=1", "policy": main_name},
+        "heuristic_gteps": round(results["heuristic"]["wall_gteps"], 3),
+        "learned_over_heuristic": (round(results["learned"]["wall_gteps"] /
+                                         results["heuristic"]["wall_gteps"], 4)
+                                   if "learned" in results else None),
+        "hm_gteps_events": {k: round(v["hm_gteps_events"], 3) for k, v in results.items()},
+        "bit_exact": bit_exact,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(g, roots, keep)
+    print(json.dumps(line), flush=True)
+    dg.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -222,6 +222,14 @@ typedef struct albba_bfs_stats {
     float *level_ms;             /* [capacity] kernel time per level (or NULL)    */
     float *features;             /* [capacity*12] selector features (or NULL)     */
     float *choice_ms;            /* [capacity*4] all-choices timings (BEST only)  */
+    int64_t *entries;            /* [capacity] pull: column entries gathered;
+                                    push: frontier edges expanded (or NULL)       */
+    int64_t *words;              /* [capacity] pull: position words scanned;
+                                    push: queue length (or NULL)                  */
+    int64_t *bypassed;           /* [capacity] pull: words skipped because every
+                                    lane is visited (slice bypass) (or NULL)      */
+    float *kernel_ms;            /* [capacity] main kernel only (pull or push
+                                    expansion), needs time_levels (or NULL)       */
 } albba_bfs_stats;
 
 /* Last error message of the calling thread (static storage). */
