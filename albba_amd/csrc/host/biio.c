@@ -254,7 +254,7 @@ int mmio_info(int *m, int *n, int *nnz, int *isSymmetric, char *filename)
     double *va;
     int rc = mm_load(filename, &mf, &ri, &ci, &va);
     if (rc) {
-        if (rc == -2) printf("Could not process Matrix Market banner.\n");
+        if (rc == -2) { printf("Could not process Matrix Market banner.\n"); fflush(stdout); }
         return rc;
     }
     int64_t total = mf.nz;
@@ -277,7 +277,7 @@ int mmio_data(int *csrRowPtr, int *csrColIdx, double *csrVal, char *filename)
     double *va;
     int rc = mm_load(filename, &mf, &ri, &ci, &va);
     if (rc) {
-        if (rc == -2) printf("Could not process Matrix Market banner.\n");
+        if (rc == -2) { printf("Could not process Matrix Market banner.\n"); fflush(stdout); }
         return rc;
     }
     const int sym = mirrored(&mf);
@@ -356,5 +356,6 @@ void read_Dmatrix(int *row, int *col, int *nnz, int **row_ptr, int **col_idx, do
         load_mtx_alloc(row, col, nnz, row_ptr, col_idx, val, isSymmetric, filename);
     } else {
         printf("[INFO]  File type unsupport\n");
+        fflush(stdout);
     }
 }
