@@ -55,7 +55,7 @@ class _SELL(C.Structure):
                 ("nvops", C.c_int64), ("row_order", I32P), ("pos_of", I32P),
                 ("slice_widths", I32P), ("slice_pointers", I64P), ("vop_pointers", I64P),
                 ("vop_lengths", U8P), ("column_indices", I32P), ("row_lengths", I32P),
-                ("push_ptr", I64P), ("push_col", I32P)]
+                ("push_ptr", I64P), ("push_col", I32P), ("pull_ptr", I64P), ("pull_col", I32P)]
 
 
 class SparseMatrixCSR(C.Structure):
@@ -315,6 +315,8 @@ class SellCS:
             "row_lengths": _view(r.row_lengths, npos, np.int32).copy(),
             "push_ptr": _view(r.push_ptr, npos + 1, np.int64).copy(),
             "push_col": _view(r.push_col, r.nnz, np.int32).copy(),
+            "pull_ptr": _view(r.pull_ptr, npos + 1, np.int64).copy() if r.pull_ptr else None,
+            "pull_col": _view(r.pull_col, r.nnz, np.int32).copy() if r.pull_col else None,
         }
 
 

@@ -16,11 +16,14 @@ extern "C" {
 #endif
 
 #define ALBD_HIDDEN __attribute__((visibility("hidden")))
-#define ALBD_NSLOT 32 /* counter slots; blocks spread their atomics over them */
+/* Counter slots: blocks spread their per-level atomics over this many 64-byte
+ * slots (same-address device atomics serialise at roughly 100 ns each), and
+ * the select kernel sums them at the end of the level. */
+#define ALBD_NSLOT 1024
 
 typedef struct albd_ctx albd_ctx;
 
-/* Per-level counters, one copy per slot; the host sums the slots. */
+/* Per-level counters, one copy per slot. */
 typedef struct albd_slot {
     unsigned long long next_count;   /* vertices discovered this level        */
     unsigned long long next_edges;   /* sum of out-degree of those vertices   */
@@ -32,18 +35,38 @@ typedef struct albd_slot {
 } albd_slot;
 
 typedef struct albd_counters {
-    albd_slot slot[ALBD_NSLOT];
     unsigned long long queue_len;    /* current frontier queue length         */
     unsigned long long qnext_len;    /* queue being built                     */
     unsigned long long active_len;   /* active (unvisited) word list length   */
     unsigned long long active_next;  /* list being built                      */
     unsigned long long push_edges;   /* sum of degrees over the queue         */
-    unsigned long long misc[3];
+    /* written by the select kernel at the end of every level */
+    unsigned long long frontier;       /* vertices found by the last level     */
+    unsigned long long frontier_edges; /* their out-edges                      */
+    unsigned long long frontier_words; /* nonzero words of their bitmap        */
+    unsigned long long reached;        /* vertices with a level so far         */
+    unsigned long long reached_edges;  /* their out-edges                      */
+    unsigned long long scanned;        /* pull: column entries gathered        */
+    unsigned long long words;          /* pull: position words scanned         */
+    unsigned long long bypassed;       /* pull: words bypassed, all visited    */
+    int sel_choice;                    /* policy's choice for the next level   */
+    int heur_choice;                   /* the reference heuristic's choice     */
+    float features[12];                /* selector features of the next level  */
 } albd_counters;
+
+/* Policy inputs of the select kernel. */
+typedef struct albd_select {
+    const float *params;   /* MLP parameters on the device, or NULL (heuristic) */
+    int32_t level;         /* index of the level being decided                  */
+    int32_t prev_choice;   /* choice committed for the previous level           */
+    int32_t valid_mask;    /* bit c set: choice c may run                        */
+    int32_t first;         /* 1 right after init: the root is the frontier       */
+} albd_select_args;
 
 /* Read-only graph arrays on the device. */
 typedef struct albd_graph {
     int32_t n, C, pad, nslices, nwords;
+    int32_t sparse_lanes;       /* pull: at most this many live lanes -> row phase */
     int64_t npos, nnz, padded_nnz, nvops;
     const int64_t *slice_ptr;   /* [nslices+1] */
     const int32_t *slice_width; /* [nslices]   */
@@ -54,6 +77,8 @@ typedef struct albd_graph {
     const int32_t *row_len;     /* [npos]      */
     const int64_t *push_ptr;    /* [npos+1]    */
     const int32_t *push_col;    /* [nnz]       */
+    const int64_t *pull_ptr;    /* [npos+1] row-major rows of the layout (may alias push) */
+    const int32_t *pull_col;    /* [nnz]       */
     const uint64_t *haslen;     /* [nwords] bit set where row_len > 0 */
 } albd_graph;
 
@@ -73,6 +98,7 @@ typedef struct albd_state {
     int32_t *active;            /* [nwords]  unvisited-word list */
     int32_t *active_next;       /* [nwords]  */
     albd_counters *ctr;         /* device counters */
+    albd_slot *slots;           /* [ALBD_NSLOT] per-level counter slots */
 } albd_state;
 
 #define ALBD_SCAN_BLOCKS 1024
@@ -132,6 +158,11 @@ ALBD_HIDDEN int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_st
 /* push: merge next into visited, write level/parent from pmin, build queue_next */
 ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                    int32_t level);
+/* end of a level: sum the slot counters into ctr->frontier*, update the
+ * running totals, compute the 12 features and the next level's choice
+ * (MLP argmin when sel->params is set, else the reference heuristic) */
+ALBD_HIDDEN int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                            const albd_select_args *sel);
 
 /* ---- selector MLP (mlp.hip); every pointer is device memory ---- */
 #define ALBD_MLP_SCRATCH_PER_SAMPLE (32 + 32 + 4 + 32 + 32 + 1)
@@ -145,9 +176,6 @@ ALBD_HIDDEN int albd_mlp_loss_grad(albd_ctx *c, const float *params, const float
                                    float *loss);
 ALBD_HIDDEN int albd_mlp_adam(albd_ctx *c, float *params, const float *grad, float *m, float *v,
                               int32_t np, float lr, float b1, float b2, float eps, int32_t step);
-/* choice[0] = argmin over choices whose bit is set in valid_mask of MLP(feat[12]) */
-ALBD_HIDDEN int albd_mlp_decide(albd_ctx *c, const float *params, const float *feat,
-                                int32_t valid_mask, int32_t *choice);
 
 #ifdef __cplusplus
 }

@@ -35,15 +35,16 @@ struct PullArgs {
     const int32_t *items;               // LIST variants: word list
     const unsigned long long *nitems;   // LIST variants: its length (device)
     int32_t sum_words32;                // CB: LDS summary size in 32-bit words
+    int32_t per_block;                  // consecutive items per block
 };
 
-__device__ __forceinline__ void slot_add(albd_counters *ctr, unsigned long long cnt,
+__device__ __forceinline__ void slot_add(albd_slot *slots, unsigned long long cnt,
                                          unsigned long long edges, unsigned long long words,
                                          unsigned long long scanned,
                                          unsigned long long words_scanned = 0,
                                          unsigned long long bypassed = 0)
 {
-    albd_slot *s = &ctr->slot[blockIdx.x % ALBD_NSLOT];
+    albd_slot *s = &slots[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % ALBD_NSLOT];
     if (cnt) atomicAdd(&s->next_count, cnt);
     if (edges) atomicAdd(&s->next_edges, edges);
     if (words) atomicAdd(&s->next_words, words);
@@ -55,13 +56,12 @@ __device__ __forceinline__ void slot_add(albd_counters *ctr, unsigned long long 
 // ---------------------------------------------------------------------------
 // pull
 
-template <int C, int VAR, int U>
+template <int C, int VAR>
 __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
 {
     extern __shared__ uint32_t lds_sum[];
     const albd_graph &g = a.g;
     const int lane = lane_id();
-    const int wib = threadIdx.x >> 6;
 
     if constexpr (VAR == ALBD_PULL_LIST_CB) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(a.st.summary);
@@ -69,69 +69,89 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
         __syncthreads();
     }
     const int64_t nitems = (VAR == ALBD_PULL_ALL) ? (int64_t)g.nwords : (int64_t)(*a.nitems);
-    // XCD-aware split: blocks are dispatched round-robin over the 8 XCDs, so
-    // blocks with the same blockIdx % 8 share an L2; give each XCD one
-    // contiguous eighth of the items so writes into a sigma window stay in
-    // one L2.
-    const int xcd = blockIdx.x & 7;
-    const int nbx = ((int)gridDim.x - xcd + 7) >> 3;
-    const int bx = blockIdx.x >> 3;
-    const int64_t lo = nitems * xcd / 8, hi = nitems * (xcd + 1) / 8;
-    const int64_t stride = (int64_t)nbx * kWavesPerBlock;
+    // Word cost is heavy-tailed (the first slice of every sigma window holds
+    // its hubs), so work is balanced twice over without global atomics: the
+    // hardware dispatcher hands out blocks of per_block consecutive words,
+    // and the block's waves take words one at a time through an LDS counter.
+    // Blocks are dispatched round-robin over the 8 XCDs (blockIdx % 8); the
+    // block -> range map is XCD-major, so each XCD sweeps its own contiguous
+    // stretch of words and the level/parent writes of a sigma window merge in
+    // one L2.  gridDim is a multiple of 8 and was sized from an upper bound of
+    // the item count, so the share of a block is taken from the real count.
+    __shared__ int next_item;
+    if (threadIdx.x == 0) next_item = 0;
+    __syncthreads();
+    const int64_t vb = (int64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    int64_t per = (nitems + gridDim.x - 1) / gridDim.x;
+    if (per < 1) per = 1;
+    const int64_t b0 = vb * per;
+    const int64_t b1 = b0 + per < nitems ? b0 + per : nitems;
     const int padm = g.pad - 1;
     constexpr int SUB = C < 64 ? C : 64;
-    const int grp = (lane / SUB) * SUB;
-    const int sl = lane % SUB;
+    const uint64_t gmask = SUB == 64 ? ~0ull : (((1ull << SUB) - 1) << ((lane / SUB) * SUB));
 
     unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0, wscan = 0, wbyp = 0;
-
-    for (int64_t it = lo + (int64_t)bx * kWavesPerBlock + wib; it < hi; it += stride) {
-        const int32_t w = (VAR == ALBD_PULL_ALL) ? (int32_t)it : a.items[it];
-        const uint64_t vis = a.st.visited[w];
-        uint64_t unres = g.haslen[w] & ~vis;
-        if (unres == 0) {
-            ++wbyp; // slice bypass: every lane with a row is visited
-            continue;
-        }
-        ++wscan;
-        const uint64_t start = unres;
-        const int64_t p = (int64_t)w * 64 + lane;
-        int s, l;
-        if constexpr (C <= 64) {
-            s = (int)(p / C);
-            l = (int)(p % C);
-        } else {
-            s = w >> 1;
-            l = ((w & 1) << 6) + lane;
-        }
-        const int width = g.slice_width[s];
-        int64_t colptr = g.slice_ptr[s];
-        const int64_t vp = g.vop_ptr[s];
-        int parent_pos = -1;
-        bool done = false;
-        for (int j0 = 0; !done; j0 += SUB) {
-            const int jj = j0 + sl;
-            const int actv = (jj < width) ? (int)g.vop_len[vp + jj] + 1 : 0;
-#pragma unroll 1
-            for (int k = 0; k < SUB; k += U) {
-                int act[U], cidx[U];
-                bool av[U], hit[U];
-                int64_t cp = colptr;
+    for (;;) {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(&next_item, 1);
+        k = __shfl(k, 0, 64);
+        const int64_t it = b0 + k;
+        if (it >= b1) break;
+        {
+            const int32_t w = (VAR == ALBD_PULL_ALL) ? (int32_t)it : a.items[it];
+            const int64_t p = (int64_t)w * 64 + lane;
+            // lane row lengths replace per-column active-lane counts: rows of
+            // a slice are sorted by length, so column j holds the lanes whose
+            // row is longer than j and its size is a ballot away
+            int s, l, len, len_o = 0;
+            if constexpr (C <= 64) {
+                s = (int)(p / C);
+                l = (int)(p % C);
+                len = g.row_len[p];
+            } else {
+                s = w >> 1;
+                l = ((w & 1) << 6) + lane;
+                len = g.row_len[p];
+                len_o = g.row_len[(int64_t)s * 128 + (((w & 1) ^ 1) << 6) + lane];
+            }
+            const uint64_t vis = a.st.visited[w];
+            int64_t cp = g.slice_ptr[s];
+            uint64_t unres = __ballot(len > 0) & ~vis;
+            if (unres == 0) {
+                ++wbyp; // slice bypass: every lane with a row is visited
+                continue;
+            }
+            ++wscan;
+            const uint64_t start = unres;
+            int parent_pos = -1;
+            int j = 0, step = 1;
+            // columns are loaded 1, 2, then 4 at a time: most lanes of a dense
+            // level hit in the first column, the survivors tend to run long
+            for (;;) {
                 const bool mine = (unres >> lane) & 1ull;
+                const uint64_t live = __ballot(mine && j < len);
+                // few live lanes left: their rows go to the row phase below
+                if (__popcll(live) <= g.sparse_lanes) break;
+                int cidx[4];
+                bool av[4], hit[4];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    act[u] = __shfl(actv, grp + k + u, 64);
-                    av[u] = mine && (l < act[u]);
-                    cidx[u] = av[u] ? __builtin_nontemporal_load(&g.colidx[cp + l]) : 0;
-                    cp += (act[u] + padm) & ~padm;
-                }
-                const uint64_t m0 = __ballot(av[0]);
-                if (m0 == 0) {
-                    done = true;
-                    break;
+                for (int u = 0; u < 4; ++u) {
+                    av[u] = false;
+                    cidx[u] = 0;
+                    if (u < step) {
+                        const int col = j + u;
+                        int act;
+                        if constexpr (C <= 64)
+                            act = __popcll(__ballot(col < len) & gmask);
+                        else
+                            act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
+                        av[u] = mine && col < len;
+                        if (av[u]) cidx[u] = g.colidx[cp + l];
+                        cp += (act + padm) & ~padm;
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     bool h = false;
                     if (av[u]) {
                         const int c = cidx[u];
@@ -143,39 +163,87 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
                     }
                     hit[u] = h;
                 }
-                scanned += __popcll(m0);
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const bool h = hit[u] && ((unres >> lane) & 1ull);
-                    if (h) parent_pos = cidx[u];
-                    unres &= ~__ballot(h);
-                    if (u > 0) scanned += __popcll(__ballot(av[u]));
+                for (int u = 0; u < 4; ++u) {
+                    if (u < step) {
+                        const bool h = hit[u] && ((unres >> lane) & 1ull);
+                        if (h) parent_pos = cidx[u];
+                        unres &= ~__ballot(h);
+                        scanned += __popcll(__ballot(av[u]));
+                    }
                 }
-                colptr = cp;
-                if (unres == 0) {
-                    done = true;
-                    break;
+                j += step;
+                if (unres == 0) break;
+                step = step < 4 ? 2 * step : 4;
+            }
+            // Row phase: the r <= sparse_lanes remaining rows (columns j..
+            // onward) are scanned from the row-major copy side by side, each
+            // by a group of G = 64/2^ceil(log2 r) lanes reading G consecutive
+            // entries per step; the first hit in row order is the min id.
+            const uint64_t rem = __ballot(((unres >> lane) & 1ull) && j < len);
+            const int r = __popcll(rem);
+            if (r) {
+                int G = 64;
+                while (G * r > 64) G >>= 1;
+                const int gid = lane / G, gl = lane % G;
+                const uint64_t gmask_r = (G == 64) ? ~0ull : (((1ull << G) - 1) << (gid * G));
+                int L = -1; // the gid-th remaining lane
+                {
+                    uint64_t t = rem;
+                    for (int k = 0; k < gid && t; ++k) t &= t - 1;
+                    if (gid < r) L = __ffsll((unsigned long long)t) - 1;
+                }
+                const int rl = __shfl(len, L < 0 ? 0 : L, 64);
+                const int64_t rs = L >= 0 ? g.pull_ptr[(int64_t)w * 64 + L] : 0;
+                int got = -1;
+                for (int k = j;; k += G) {
+                    const bool live_g = L >= 0 && got < 0 && k < rl;
+                    if (__ballot(live_g) == 0) break;
+                    const int e = k + gl;
+                    bool h = false;
+                    int c = 0;
+                    if (live_g && e < rl) {
+                        c = g.pull_col[rs + e];
+                        const int wd = c >> 6;
+                        bool maybe = true;
+                        if constexpr (VAR == ALBD_PULL_LIST_CB)
+                            maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                        if (maybe) h = (a.st.front[wd] >> (c & 63)) & 1ull;
+                    }
+                    scanned += __popcll(__ballot(live_g && e < rl));
+                    const uint64_t hm = __ballot(h) & gmask_r;
+                    const int src = hm ? __ffsll((unsigned long long)hm) - 1 : lane;
+                    const int first = __shfl(c, src, 64);
+                    if (hm && got < 0) got = first;
+                }
+                for (int q = 0; q < r; ++q) {
+                    const int Lq = __shfl(L, q * G, 64);
+                    const int gq = __shfl(got, q * G, 64);
+                    if (gq >= 0) {
+                        unres &= ~(1ull << Lq);
+                        if (lane == Lq) parent_pos = gq;
+                    }
                 }
             }
-        }
-        const uint64_t found = start & ~unres;
-        if (found) {
-            if (lane == 0) {
-                a.st.next[w] = found;
-                a.st.visited[w] = vis | found;
+            const uint64_t found = start & ~unres;
+            if (found) {
+                if (lane == 0) {
+                    a.st.next[w] = found;
+                    a.st.visited[w] = vis | found;
+                }
+                if ((found >> lane) & 1ull) {
+                    const int o = g.row_order[p];
+                    a.st.level[o] = a.level + 1;
+                    a.st.parent[o] = g.row_order[parent_pos];
+                    edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
+                }
+                cnt += __popcll(found);
+                nzw += 1;
             }
-            if ((found >> lane) & 1ull) {
-                const int o = g.row_order[p];
-                a.st.level[o] = a.level + 1;
-                a.st.parent[o] = g.row_order[parent_pos];
-                edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
-            }
-            cnt += __popcll(found);
-            nzw += 1;
         }
     }
     edges = wave_sum(edges);
-    if (lane == 0) slot_add(a.st.ctr, cnt, edges, nzw, scanned, wscan, wbyp);
+    if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,6 +276,10 @@ __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, 
             st.next[i] = 0;
         }
     }
+    unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+         i < (int64_t)(ALBD_NSLOT * sizeof(albd_slot) / 8); i += (int64_t)gridDim.x * kBlock)
+        slots[i] = 0;
     if (blockIdx.x == 0) {
         unsigned long long *c = reinterpret_cast<unsigned long long *>(st.ctr);
         for (int i = threadIdx.x; i < (int)(sizeof(albd_counters) / 8); i += kBlock) c[i] = 0;
@@ -221,11 +293,112 @@ __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
     const uint64_t bit = 1ull << (rp & 63);
     st.visited[rp >> 6] = bit;
     st.front[rp >> 6] = bit;
-    if (g.push_ptr[rp + 1] > g.push_ptr[rp]) {
+    const int64_t deg = g.push_ptr[rp + 1] - g.push_ptr[rp];
+    if (deg > 0) {
         st.queue[0] = rp;
         st.ctr->queue_len = 1;
     } else {
         st.ctr->queue_len = 0;
+    }
+    // the root is the frontier of "level -1": seed the slot counters the
+    // select kernel sums
+    st.slots[0].next_count = 1;
+    st.slots[0].next_edges = (unsigned long long)deg;
+    st.slots[0].next_words = 1;
+}
+
+// End of a level: the frontier it produced, the running totals, the
+// selector features and the policy's choice for the next level.  One wave.
+// Features (all roughly in [0,1]), with F = frontier vertices, mf = their
+// out-edges, m = stored entries:
+//   0 log2(1+F)/log2(1+n)   1 F/n   2 log2(1+mf)/log2(1+m)   3 mf/m
+//   4 unvisited vertices/n  5 unvisited edges/m  6 nonzero frontier words/words
+//   7 min(level,64)/64      8 log2(1+mf/F)/16    9 log2(1+m/n)/8
+//   10 log2(n)/32           11 previous level pushed (1) or pulled (0)
+__global__ __launch_bounds__(64) void k_select(albd_graph g, albd_state st, albd_select_args sel)
+{
+    __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
+    const int lane = lane_id();
+    unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
+    for (int i = lane; i < ALBD_NSLOT; i += 64) {
+        const albd_slot &sl = st.slots[i];
+        F += sl.next_count;
+        mf += sl.next_edges;
+        nzw += sl.next_words;
+        scn += sl.scanned;
+        wds += sl.words;
+        byp += sl.bypassed;
+    }
+    F = wave_sum(F);
+    mf = wave_sum(mf);
+    nzw = wave_sum(nzw);
+    scn = wave_sum(scn);
+    wds = wave_sum(wds);
+    byp = wave_sum(byp);
+    const unsigned long long reached = st.ctr->reached + F;
+    const unsigned long long redges = st.ctr->reached_edges + mf;
+    const double n = (double)g.n, m = g.nnz > 0 ? (double)g.nnz : 1.0;
+    if (lane == 0) {
+        xs[0] = (float)(log2(1.0 + (double)F) / log2(1.0 + n));
+        xs[1] = (float)((double)F / n);
+        xs[2] = (float)(log2(1.0 + (double)mf) / log2(1.0 + m));
+        xs[3] = (float)((double)mf / m);
+        xs[4] = (float)((n - (double)reached) / n);
+        xs[5] = (float)((m - (double)redges) / m);
+        xs[6] = (float)((double)nzw / (double)(g.nwords > 0 ? g.nwords : 1));
+        xs[7] = (float)((sel.level < 64 ? sel.level : 64) / 64.0);
+        xs[8] = (float)(log2(1.0 + (double)mf / (double)(F > 0 ? F : 1)) / 16.0);
+        xs[9] = (float)(log2(1.0 + m / n) / 8.0);
+        xs[10] = (float)(log2(n) / 32.0);
+        xs[11] = sel.prev_choice == ALBBA_CHOICE_PUSH ? 1.0f : 0.0f;
+    }
+    __syncthreads();
+    int choice = -1;
+    if (sel.params) {
+        // W1[32][12] b1[32] W2[32][32] b2[32] W3[4][32] b3[4]: lane j owns unit j
+        const float *W1 = sel.params, *B1 = W1 + 32 * 12, *W2 = B1 + 32, *B2 = W2 + 32 * 32,
+                    *W3 = B2 + 32, *B3 = W3 + 4 * 32;
+        if (lane < ALBBA_MLP_H1) {
+            float a = B1[lane];
+            for (int i = 0; i < ALBBA_MLP_IN; ++i) a = fmaf(W1[lane * ALBBA_MLP_IN + i], xs[i], a);
+            h1[lane] = a > 0.f ? a : 0.f;
+        }
+        __syncthreads();
+        if (lane < ALBBA_MLP_H2) {
+            float a = B2[lane];
+            for (int i = 0; i < ALBBA_MLP_H1; ++i) a = fmaf(W2[lane * ALBBA_MLP_H1 + i], h1[i], a);
+            h2[lane] = a > 0.f ? a : 0.f;
+        }
+        __syncthreads();
+        if (lane < ALBBA_MLP_OUT) {
+            float a = B3[lane];
+            for (int i = 0; i < ALBBA_MLP_H2; ++i) a = fmaf(W3[lane * ALBBA_MLP_H2 + i], h2[i], a);
+            ys[lane] = a;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            float bv = 0.f;
+            for (int k = 0; k < ALBBA_MLP_OUT; ++k) {
+                if (!((sel.valid_mask >> k) & 1) || !(ys[k] == ys[k])) continue; // NaN never wins
+                if (choice < 0 || ys[k] < bv) { choice = k; bv = ys[k]; }
+            }
+        }
+    }
+    if (lane == 0) {
+        // bfs/src/bfs_SELLCSv2.c:282-284: pull when F > (int)(n * 0.04)
+        const long long thr = (long long)(int)(n * 0.04);
+        const int heur = (long long)F > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
+        st.ctr->frontier = F;
+        st.ctr->frontier_edges = mf;
+        st.ctr->frontier_words = nzw;
+        st.ctr->scanned = scn;
+        st.ctr->words = wds;
+        st.ctr->bypassed = byp;
+        st.ctr->reached = reached;
+        st.ctr->reached_edges = redges;
+        st.ctr->heur_choice = heur;
+        st.ctr->sel_choice = choice >= 0 ? choice : heur;
+        for (int i = 0; i < ALBBA_MLP_IN; ++i) st.ctr->features[i] = xs[i];
     }
 }
 
@@ -236,9 +409,11 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords;
          i += (int64_t)gridDim.x * kBlock)
         st.next[i] = 0;
+    unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+         i < (int64_t)(ALBD_NSLOT * sizeof(albd_slot) / 8); i += (int64_t)gridDim.x * kBlock)
+        slots[i] = 0;
     if (blockIdx.x == 0) {
-        unsigned long long *c = reinterpret_cast<unsigned long long *>(st.ctr->slot);
-        for (int i = threadIdx.x; i < (int)(sizeof(st.ctr->slot) / 8); i += kBlock) c[i] = 0;
         if (threadIdx.x == 0) {
             if (flags & LB_QUEUE_FROM_NEXT) st.ctr->queue_len = st.ctr->qnext_len;
             if (flags & LB_ZERO_QUEUE) st.ctr->queue_len = 0;
@@ -357,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
         cnt = wave_sum(cnt);
         nzw = wave_sum(nzw);
         edges = wave_sum(edges);
-        if (lane == 0) slot_add(st.ctr, cnt, edges, nzw, 0);
+        if (lane == 0) slot_add(st.slots, cnt, edges, nzw, 0);
     }
 }
 
@@ -533,20 +708,30 @@ int grid_for_groups(albd_ctx *c, int64_t groups)
 }
 
 template <int C>
-int launch_pull_c(albd_ctx *c, const PullArgs &a, int variant, size_t lds)
+int launch_pull_c(albd_ctx *c, PullArgs a, int variant, size_t lds, int64_t items)
 {
     hipStream_t s = albd_stream(c);
-    constexpr int U = 2;
-    if (variant == ALBD_PULL_ALL) {
-        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_ALL, U>), dim3(grid_persistent(c, 8)), dim3(kBlock), 0, s, a);
-    } else if (variant == ALBD_PULL_LIST) {
-        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST, U>), dim3(grid_persistent(c, 8)), dim3(kBlock), 0, s, a);
-    } else {
+    // blocks of per_block words; the grid is a multiple of 8 (one share per XCD)
+    int64_t blocks;
+    if (variant == ALBD_PULL_LIST_CB) {
+        // the LDS summary is staged once per block: fewer, longer blocks
         int per_cu = (int)(160 * 1024 / (lds + 1024));
-        if (per_cu > 8) per_cu = 8;
-        if (per_cu < 1) per_cu = 1;
-        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST_CB, U>), dim3(grid_persistent(c, per_cu)), dim3(kBlock), lds, s, a);
+        per_cu = per_cu > 8 ? 8 : (per_cu < 1 ? 1 : per_cu);
+        const int64_t resident = (int64_t)albd_num_cus(c) * per_cu;
+        a.per_block = (int32_t)((items + resident - 1) / resident);
+        if (a.per_block < 16) a.per_block = 16;
+    } else {
+        a.per_block = 16;
     }
+    blocks = (items + a.per_block - 1) / a.per_block;
+    blocks = (blocks + 7) / 8 * 8;
+    if (blocks < 8) blocks = 8;
+    if (variant == ALBD_PULL_ALL)
+        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_ALL>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    else if (variant == ALBD_PULL_LIST)
+        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST_CB>), dim3((unsigned)blocks), dim3(kBlock), lds, s, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -625,13 +810,13 @@ int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st)
 int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant, int32_t level,
               int64_t items_bound)
 {
-    (void)items_bound;
     PullArgs a;
     a.g = *g;
     a.st = *st;
     a.level = level;
     a.items = st->active_next;
     a.nitems = &st->ctr->active_next;
+    a.per_block = 16;
     const int64_t sum_words64 = ((int64_t)g->nwords + 63) / 64;
     a.sum_words32 = (int32_t)(sum_words64 * 2);
     size_t lds = 0;
@@ -639,11 +824,14 @@ int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int varian
         if (!albd_cb_supported(g)) return albd_set_error_msg(ALBBA_EINVAL, "column bypass: graph too large for the LDS summary");
         lds = (size_t)a.sum_words32 * 4;
     }
+    // items: every word, or an upper bound of the list length the device holds
+    int64_t items = g->nwords;
+    if (variant != ALBD_PULL_ALL && items_bound > 0 && items_bound < items) items = items_bound;
     switch (g->C) {
-    case 16: return launch_pull_c<16>(c, a, variant, lds);
-    case 32: return launch_pull_c<32>(c, a, variant, lds);
-    case 64: return launch_pull_c<64>(c, a, variant, lds);
-    case 128: return launch_pull_c<128>(c, a, variant, lds);
+    case 16: return launch_pull_c<16>(c, a, variant, lds, items);
+    case 32: return launch_pull_c<32>(c, a, variant, lds, items);
+    case 64: return launch_pull_c<64>(c, a, variant, lds, items);
+    case 128: return launch_pull_c<128>(c, a, variant, lds, items);
     default: return albd_set_error_msg(ALBBA_EINVAL, "pull: unsupported slice height");
     }
 }
@@ -676,6 +864,13 @@ int albd_push_scan(albd_ctx *c, const albd_graph *g, const albd_state *st, int64
 int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
     hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, albd_stream(c), *g, *st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st, const albd_select_args *sel)
+{
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, albd_stream(c), *g, *st, *sel);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -146,24 +146,6 @@ __global__ __launch_bounds__(256) void k_adam(float *params, const float *grad, 
     params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
-__global__ void k_decide(const float *params, const float *feat, int32_t valid_mask, int32_t *choice)
-{
-    __shared__ float sp[NP];
-    load_params(params, sp);
-    if (threadIdx.x != 0) return;
-    float xi[NI], h1[H1], h2[H2], y[NO];
-    for (int i = 0; i < NI; ++i) xi[i] = feat[i];
-    forward_one(sp, xi, h1, h2, y);
-    int best = -1;
-    float bv = 0.f;
-    for (int k = 0; k < NO; ++k) {
-        if (!((valid_mask >> k) & 1)) continue;
-        if (!(y[k] == y[k])) continue; // NaN: not a candidate
-        if (best < 0 || y[k] < bv) { best = k; bv = y[k]; }
-    }
-    *choice = best;
-}
-
 } // namespace
 
 extern "C" {
@@ -194,14 +176,6 @@ int albd_mlp_adam(albd_ctx *c, float *params, const float *grad, float *m, float
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, albd_stream(c), params, grad, m, v,
                        np, lr, b1, b2, eps, c1, c2);
-    HIPCHK(hipGetLastError());
-    return 0;
-}
-
-int albd_mlp_decide(albd_ctx *c, const float *params, const float *feat, int32_t valid_mask,
-                    int32_t *choice)
-{
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, albd_stream(c), params, feat, valid_mask, choice);
     HIPCHK(hipGetLastError());
     return 0;
 }

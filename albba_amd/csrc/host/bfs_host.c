@@ -37,8 +37,6 @@ struct albba_graph {
     int64_t device_bytes;
     void *scratch_visited; /* BEST-policy snapshots */
     void *scratch_ctr;
-    void *feat_dev;        /* [12] features of the level being decided */
-    void *choice_dev;      /* int32 choice written by the select kernel */
 };
 
 /* ---- context ---- */
@@ -128,6 +126,10 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, (void *)G->g.colidx);
     albd_free(d, (void *)G->g.row_order);
     albd_free(d, (void *)G->g.row_len);
+    if (G->g.pull_ptr != G->g.push_ptr) {
+        albd_free(d, (void *)G->g.pull_ptr);
+        albd_free(d, (void *)G->g.pull_col);
+    }
     albd_free(d, (void *)G->g.push_ptr);
     albd_free(d, (void *)G->g.push_col);
     albd_free(d, (void *)G->g.haslen);
@@ -145,10 +147,9 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, G->st.active);
     albd_free(d, G->st.active_next);
     albd_free(d, G->st.ctr);
+    albd_free(d, G->st.slots);
     albd_free(d, G->scratch_visited);
     albd_free(d, G->scratch_ctr);
-    albd_free(d, G->feat_dev);
-    albd_free(d, G->choice_dev);
     free(G->pos_of);
     free(G);
 }
@@ -169,6 +170,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     g->n = s->n;
     g->C = s->C;
     g->pad = s->pad;
+    g->sparse_lanes = 8;
     g->nslices = s->nslices;
     g->nwords = (int32_t)nwords;
     g->npos = npos;
@@ -185,6 +187,13 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     g->row_len = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
     g->push_ptr = (const int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
     g->push_col = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->nnz > 0 ? s->nnz : 1), &rc);
+    if (s->pull_ptr) {
+        g->pull_ptr = (const int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
+        g->pull_col = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->nnz > 0 ? s->nnz : 1), &rc);
+    } else {
+        g->pull_ptr = g->push_ptr;
+        g->pull_col = g->push_col;
+    }
     g->haslen = (const uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     albd_state *st = &G->st;
     const int64_t nsum = (nwords + 63) / 64 + 1;
@@ -202,10 +211,9 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
     st->active_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
     st->ctr = (albd_counters *)dalloc(G, sizeof(albd_counters), &rc);
+    st->slots = (albd_slot *)dalloc(G, sizeof(albd_slot) * ALBD_NSLOT, &rc);
     G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
-    G->feat_dev = dalloc(G, sizeof(float) * 16, &rc);
-    G->choice_dev = dalloc(G, sizeof(int32_t) * 4, &rc);
     if (rc) {
         int code = rc;
         const char *msg = albd_error();
@@ -224,6 +232,8 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         (rc = upload_padded_i32(ctx, (int32_t *)g->row_len, s->row_lengths, npos_raw, npos, 0)) ||
         (rc = upload_padded_i64(ctx, (int64_t *)g->push_ptr, s->push_ptr, npos_raw + 1, npos + 1)) ||
         (rc = albd_h2d(ctx->d, (void *)g->push_col, s->push_col, sizeof(int32_t) * (size_t)s->nnz)) ||
+        (s->pull_ptr && (rc = upload_padded_i64(ctx, (int64_t *)g->pull_ptr, s->pull_ptr, npos_raw + 1, npos + 1))) ||
+        (s->pull_ptr && (rc = albd_h2d(ctx->d, (void *)g->pull_col, s->pull_col, sizeof(int32_t) * (size_t)s->nnz))) ||
         (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
         (rc = albd_sync(ctx->d))) {
         char buf[512];
@@ -269,26 +279,17 @@ typedef struct level_sums {
 
 static void sum_counters(const albd_counters *h, level_sums *o)
 {
-    memset(o, 0, sizeof(*o));
-    for (int i = 0; i < ALBD_NSLOT; ++i) {
-        o->next_count += (int64_t)h->slot[i].next_count;
-        o->next_edges += (int64_t)h->slot[i].next_edges;
-        o->next_words += (int64_t)h->slot[i].next_words;
-        o->scanned += (int64_t)h->slot[i].scanned;
-        o->words += (int64_t)h->slot[i].words;
-        o->bypassed += (int64_t)h->slot[i].bypassed;
-    }
+    /* the select kernel has summed the slot counters of the level */
+    o->next_count = (int64_t)h->frontier;
+    o->next_edges = (int64_t)h->frontier_edges;
+    o->next_words = (int64_t)h->frontier_words;
+    o->scanned = (int64_t)h->scanned;
+    o->words = (int64_t)h->words;
+    o->bypassed = (int64_t)h->bypassed;
     o->qnext_len = (int64_t)h->qnext_len;
     o->active_next = (int64_t)h->active_next;
     o->queue_len = (int64_t)h->queue_len;
     o->push_edges = (int64_t)h->push_edges;
-}
-
-int albba_heuristic_choice(int32_t n, int64_t frontier)
-{
-    /* bfs/src/bfs_SELLCSv2.c:282-284: int threshold = n * 0.04 (truncated) */
-    const int thr = (int)(n * 0.04);
-    return frontier > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
 }
 
 /* Launch the kernels of one level (no pointer swaps).  With `mark` set, HIP
@@ -319,7 +320,9 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
         return rc;
     if (variant == ALBD_PULL_LIST_CB && (rc = albd_build_summary(d, &G->g, &G->st))) return rc;
     if (mark && (rc = albd_event_record(d, 6))) return rc;
-    if ((rc = albd_pull(d, &G->g, &G->st, variant, level, 0))) return rc;
+    /* a rebuilt list is never longer than the list it was compacted from */
+    const int64_t bound = ls->list_valid ? ls->list_len : G->g.nwords;
+    if ((rc = albd_pull(d, &G->g, &G->st, variant, level, bound))) return rc;
     return mark ? albd_event_record(d, 7) : 0;
 }
 
@@ -384,36 +387,33 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
     albd_ctx *d = ctx->d;
     const int reps = opts.timing_reps > 0 ? opts.timing_reps : 1;
     const int32_t rp = G->pos_of[root];
-    /* root out-degree for the Graph500 edge count */
-    int64_t pp[2];
-    int rc = albd_d2h(d, pp, G->g.push_ptr + rp, sizeof pp);
-    if (rc) return albba_fail(rc, "bfs: %s", albd_error());
+    int valid_mask = 0;
+    for (int c = 0; c < ALBBA_NUM_CHOICES; ++c) valid_mask |= choice_ok(G, c) << c;
+    albd_select_args sel = {policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(opts.selector) : NULL,
+                       0, ALBBA_CHOICE_PUSH, valid_mask, 1};
 
     loop_state ls = {1, 0, 0, 0, 0};
-    int64_t frontier = 1, frontier_edges = pp[1] - pp[0], nzwords = 1;
-    int64_t reached = 1, reached_edges = pp[1] - pp[0];
     int32_t level = 0;
-    int prev_choice = ALBBA_CHOICE_PUSH;
-    float feats[ALBBA_NUM_FEATURES];
-
-    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
+    int rc;
+    /* init, then the select kernel turns the root into level 0's frontier */
+    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)) ||
+        (rc = albd_select(d, &G->g, &G->st, &sel)) || (rc = albd_read_counters(d, &G->st)))
         return albba_fail(rc, "bfs: %s", albd_error());
+    const albd_counters *hc = albd_host_counters(d);
+    int64_t frontier = (int64_t)hc->frontier;
 
     while (frontier > 0) {
-        albba_features(G->n, G->nnz, G->g.nwords, frontier, frontier_edges, nzwords, reached,
-                       reached_edges, level, prev_choice, feats);
-        int choice = albba_heuristic_choice(G->n, frontier);
+        float feats[ALBBA_NUM_FEATURES];
+        memcpy(feats, hc->features, sizeof feats);
+        /* heuristic and learned choices come from the select kernel (the
+         * learned one already falls back to the heuristic) */
+        int choice = policy == ALBBA_SWITCH_LEARNED ? hc->sel_choice : hc->heur_choice;
         float choice_ms[ALBBA_NUM_CHOICES] = {0, 0, 0, 0};
         if (policy == ALBBA_SWITCH_FIXED) {
             choice = opts.fixed_choice;
         } else if (policy == ALBBA_SWITCH_SCRIPT && opts.script && level < opts.script_len &&
                    choice_ok(G, opts.script[level])) {
             choice = opts.script[level];
-        } else if (policy == ALBBA_SWITCH_LEARNED) {
-            int c2 = -1;
-            rc = albba_selector_decide(opts.selector, d, feats, G->feat_dev, G->choice_dev, &c2);
-            if (rc) return albba_fail(rc, "bfs: selector: %s", albd_error());
-            if (choice_ok(G, c2)) choice = c2; /* else: heuristic fallback */
         } else if (policy == ALBBA_SWITCH_BEST) {
             /* time every choice on a snapshot of the state, commit the fastest */
             if ((rc = albd_d2d(d, G->scratch_visited, G->st.visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
@@ -446,9 +446,13 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
         if ((rc = run_level(G, &ls, choice, level, frontier, opts.time_levels)))
             return albba_fail(rc, "bfs: %s", albd_error());
         if (opts.time_levels && (rc = albd_event_record(d, 3))) return albba_fail(rc, "bfs: %s", albd_error());
-        if ((rc = albd_read_counters(d, &G->st))) return albba_fail(rc, "bfs: %s", albd_error());
+        sel.level = level + 1;
+        sel.prev_choice = choice;
+        sel.first = 0;
+        if ((rc = albd_select(d, &G->g, &G->st, &sel)) || (rc = albd_read_counters(d, &G->st)))
+            return albba_fail(rc, "bfs: %s", albd_error());
         level_sums sums;
-        sum_counters(albd_host_counters(d), &sums);
+        sum_counters(hc, &sums);
         if (stats && level < stats->capacity) {
             if (stats->frontier) stats->frontier[level] = (int32_t)frontier;
             if (stats->choice) stats->choice[level] = choice;
@@ -470,12 +474,7 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
             if (stats->bypassed) stats->bypassed[level] = push ? 0 : sums.bypassed;
         }
         commit_level(G, &ls, choice, &sums);
-        prev_choice = choice;
-        frontier = sums.next_count;
-        frontier_edges = sums.next_edges;
-        nzwords = sums.next_words;
-        reached += sums.next_count;
-        reached_edges += sums.next_edges;
+        frontier = (int64_t)hc->frontier;
         ++level;
     }
     float ms = 0;
@@ -483,8 +482,8 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
         return albba_fail(rc, "bfs: %s", albd_error());
     if (stats) {
         stats->num_levels = level;
-        stats->reached = reached;
-        stats->edges_in_component = reached_edges;
+        stats->reached = (int64_t)hc->reached;
+        stats->edges_in_component = (int64_t)hc->reached_edges;
         stats->bfs_ms = ms;
     }
     return ALBBA_OK;

@@ -20,25 +20,6 @@ struct albba_selector {
     float params[ALBBA_MLP_NPARAM];
 };
 
-void albba_features(int32_t n, int64_t m, int32_t nwords, int64_t F, int64_t mf, int64_t nzw,
-                    int64_t reached, int64_t reached_edges, int32_t level, int prev_choice,
-                    float *o)
-{
-    const double dn = n > 0 ? (double)n : 1.0, dm = m > 0 ? (double)m : 1.0;
-    o[0] = (float)(log2(1.0 + (double)F) / log2(1.0 + dn));
-    o[1] = (float)((double)F / dn);
-    o[2] = (float)(log2(1.0 + (double)mf) / log2(1.0 + dm));
-    o[3] = (float)((double)mf / dm);
-    o[4] = (float)((double)(n - reached) / dn);
-    o[5] = (float)((double)(m - reached_edges) / dm);
-    o[6] = (float)((double)nzw / (double)(nwords > 0 ? nwords : 1));
-    o[7] = (float)((level < 64 ? level : 64) / 64.0);
-    o[8] = (float)(log2(1.0 + (double)mf / (double)(F > 0 ? F : 1)) / 16.0);
-    o[9] = (float)(log2(1.0 + dm / dn) / 8.0);
-    o[10] = (float)(log2(dn) / 32.0);
-    o[11] = prev_choice == ALBBA_CHOICE_PUSH ? 1.0f : 0.0f;
-}
-
 static int upload(albba_selector *s)
 {
     albd_ctx *d = albba_ctx_dev(s->ctx);
@@ -208,13 +189,4 @@ int albba_selector_predict(albba_selector *s, const float *x, int32_t ns, float 
     return rc ? albba_fail(rc, "selector_predict: %s", albd_error()) : ALBBA_OK;
 }
 
-int albba_selector_decide(const albba_selector *s, albd_ctx *d, const float *feats, void *feat_dev,
-                          void *choice_dev, int *choice)
-{
-    int rc = albd_h2d(d, feat_dev, feats, sizeof(float) * ALBBA_NUM_FEATURES);
-    int32_t c = -1;
-    if (!rc) rc = albd_mlp_decide(d, s->dparams, (const float *)feat_dev, 0xF, (int32_t *)choice_dev);
-    if (!rc) rc = albd_d2h(d, &c, choice_dev, sizeof(int32_t));
-    *choice = c;
-    return rc;
-}
+const float *albba_selector_dparams(const albba_selector *s) { return s ? s->dparams : NULL; }

@@ -248,6 +248,8 @@ void albba_sellcs_free(albba_sellcs *s)
     free(s->row_lengths);
     free(s->push_ptr);
     free(s->push_col);
+    free(s->pull_ptr);
+    free(s->pull_col);
     free(s);
 }
 
@@ -405,6 +407,18 @@ int albba_sellcs_build(const albba_csr *g, int C, int sigma, int pad, albba_sell
             for (int64_t k = g->row_ptr[v]; k < g->row_ptr[v + 1]; ++k) s->push_col[cur[s->pos_of[gcol[k]]]++] = p;
         }
         free(cur);
+        /* row-major copy of the layout's own rows for the pull's long rows */
+        s->pull_ptr = (int64_t *)albba_calloc((size_t)npos + 1, sizeof(int64_t));
+        s->pull_col = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(g->nnz > 0 ? g->nnz : 1));
+        if (!s->pull_ptr || !s->pull_col) { free(sorted_copy); albba_sellcs_free(s); return albba_fail(ALBBA_ENOMEM, "sellcs_build: pull csr"); }
+        for (int64_t p = 0; p < npos; ++p)
+            s->pull_ptr[p + 1] = s->pull_ptr[p] + (p < n ? s->row_lengths[p] : 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int32_t p = 0; p < n; ++p) {
+            int32_t v = s->row_order[p];
+            int64_t o = s->pull_ptr[p];
+            for (int64_t k = g->row_ptr[v]; k < g->row_ptr[v + 1]; ++k) s->pull_col[o++] = s->pos_of[gcol[k]];
+        }
     }
     free(sorted_copy);
     *out = s;

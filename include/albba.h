@@ -174,6 +174,11 @@ typedef struct albba_sellcs {
     /* push (out-edge) CSR over positions: transpose of the layout's matrix */
     int64_t *push_ptr;        /* [n+1]                                      */
     int32_t *push_col;        /* [nnz]                                      */
+    /* row-major copy of the layout's rows over positions (in-neighbours in
+     * ascending id), read by the pull for long rows; NULL when the matrix is
+     * symmetric, where the push CSR holds the same rows */
+    int64_t *pull_ptr;        /* [n+1] or NULL                              */
+    int32_t *pull_col;        /* [nnz] or NULL                              */
 } albba_sellcs;
 
 typedef struct albba_ctx albba_ctx;       /* one device + one stream           */

@@ -195,6 +195,14 @@ def test_device_layout_is_the_reference_layout_renumbered(name, C_, pad):
     pairs_w = {(int(s["pos_of"][c]), int(s["pos_of"][v])) for v in range(n)
                for c in col[rp[v]:rp[v + 1]]}
     assert pairs_push == pairs_w
+    # rows of the layout, row-major, in ascending vertex id: the pull CSR, or
+    # the push CSR itself when the matrix is symmetric
+    rptr, rcol = (s["pull_ptr"], s["pull_col"]) if s["pull_ptr"] is not None else (pp, pc)
+    if name.startswith("dir_"):
+        assert s["pull_ptr"] is not None
+    for p in range(n):
+        v = int(s["row_order"][p])
+        np.testing.assert_array_equal(rcol[rptr[p]:rptr[p + 1]], s["pos_of"][col[rp[v]:rp[v + 1]]])
 
 
 def test_device_layout_rejects_bad_parameters():
