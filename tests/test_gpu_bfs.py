@@ -175,3 +175,27 @@ def test_grid_many_levels(ctx):
         assert o["num_levels"] > 300
         check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PULL_CB, capacity=1 << 14)
     dg.close()
+
+
+def test_cli_driver_replays_the_reference_sequence(tmp_path):
+    """albba_bfs -m: read_Dmatrix -> convert_csr_to_csc -> run_custom_bfs_test
+    through the reference entry points only; frontier sizes equal the
+    reference's run_bfs_ref counts on the transposed matrix."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(A.__file__), "albba_bfs")
+    for name in ("general_real", "mixed_case_comments"):
+        out = subprocess.run([exe, "-m", os.path.join(HERE, "golden", "mtx", name + ".mtx")],
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        line = next(ln for ln in out.stdout.splitlines() if ln.startswith("frontier:"))
+        got = np.array([int(x) for x in line.split()[1:]])
+        np.testing.assert_array_equal(got, GG[f"dir_{name}__bfs_ref_frontier"])
+    g = A.Graph.rmat(12, 16, seed=3, perm_seed=4)
+    path = str(tmp_path / "g.cbd")
+    g.write_cbd(path)
+    roots = ",".join(str(r) for r in g.pick_roots(4, 1))
+    out = subprocess.run([exe, "-m", path, "-r", roots], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.count("tree ok") == 4
