@@ -85,7 +85,8 @@ class SparseMatrixSELLCS(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("switch_policy", C.c_int32), ("fixed_choice", C.c_int32), ("script", I32P),
                 ("script_len", C.c_int32), ("selector", C.c_void_p), ("time_levels", C.c_int32),
-                ("timing_reps", C.c_int32)]
+                ("timing_reps", C.c_int32), ("small_frontier_edges", C.c_int64),
+                ("small_frontier_blocks", C.c_int32), ("reserved_", C.c_int32)]
 
 
 class _Stats(C.Structure):
@@ -127,6 +128,7 @@ def lib() -> C.CDLL:
         "albba_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
         "albba_ctx_destroy": (None, [C.c_void_p]),
         "albba_ctx_synchronize": (C.c_int, [C.c_void_p]),
+        "albba_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_float)]),
         "albba_graph_upload": (C.c_int, [C.c_void_p, C.POINTER(_SELL), C.POINTER(C.c_void_p)]),
         "albba_graph_free": (None, [C.c_void_p]),
         "albba_graph_device_bytes": (C.c_int64, [C.c_void_p]),
@@ -347,6 +349,12 @@ class Context:
     def synchronize(self):
         _check(lib().albba_ctx_synchronize(self._p))
 
+    def bench_copy(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of one device-to-device copy of nbytes (copy kernel)."""
+        ms = C.c_float()
+        _check(lib().albba_bench_copy(self._p, int(nbytes), int(reps), C.byref(ms)))
+        return float(ms.value)
+
     def close(self):
         for ref in getattr(self, "_children", []):
             obj = ref()
@@ -384,14 +392,22 @@ class DeviceGraph:
 
     def bfs(self, root: int, policy: int = SWITCH_HEURISTIC, fixed_choice: int = CHOICE_PULL_SB,
             script=None, selector: "Selector" = None, time_levels: bool = False,
-            timing_reps: int = 3, capacity: int = 0, results: bool = True) -> dict:
-        """Run one BFS on the device; returns stats (+ level/parent arrays)."""
+            timing_reps: int = 3, capacity: int = 0, results: bool = True,
+            small_frontier_edges: int = None, small_frontier_blocks: int = 0) -> dict:
+        """Run one BFS on the device; returns stats (+ level/parent arrays).
+
+        small_frontier_edges: edge cap of the cooperative small-frontier push
+        levels (0 disables them; None keeps the library default);
+        small_frontier_blocks: its workgroup count (0: default)."""
         o = _Opts()
         lib().albba_bfs_opts_default(C.byref(o))
         o.switch_policy = policy
         o.fixed_choice = fixed_choice
         o.time_levels = int(time_levels)
         o.timing_reps = timing_reps
+        if small_frontier_edges is not None:
+            o.small_frontier_edges = int(small_frontier_edges)
+        o.small_frontier_blocks = int(small_frontier_blocks)
         scr = None
         if script is not None:
             scr = np.ascontiguousarray(script, dtype=np.int32)

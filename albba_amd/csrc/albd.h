@@ -52,7 +52,22 @@ typedef struct albd_counters {
     int sel_choice;                    /* policy's choice for the next level   */
     int heur_choice;                   /* the reference heuristic's choice     */
     float features[12];                /* selector features of the next level  */
+    int persist_levels;                /* levels run by the small-frontier kernel */
+    int persist_in_queue;              /* 1: its last frontier is in st.queue   */
+    int persist_error;                 /* 1: its grid barrier gave up           */
+    int pad_;
 } albd_counters;
+
+/* levels one small-frontier call may run (st.level_log holds 4 per level) */
+#define ALBD_PERSIST_LEVELS 4096
+
+/* grid state of the small-frontier kernel, zeroed before each launch */
+typedef struct albd_persist {
+    unsigned long long found_total;  /* vertices found so far by the launch */
+    unsigned long long barrier;      /* arrivals at the grid barrier        */
+    unsigned int error;              /* a workgroup gave up waiting         */
+    unsigned int pad_;
+} albd_persist;
 
 /* Policy inputs of the select kernel. */
 typedef struct albd_select {
@@ -99,6 +114,9 @@ typedef struct albd_state {
     int32_t *active_next;       /* [nwords]  */
     albd_counters *ctr;         /* device counters */
     albd_slot *slots;           /* [ALBD_NSLOT] per-level counter slots */
+    int64_t *level_log;         /* [4*ALBD_PERSIST_LEVELS] found, edges, queue per level */
+    unsigned long long *pkey;   /* [npos] small-frontier (level, parent) keys, all ones at rest */
+    albd_persist *persist;      /* small-frontier grid state */
 } albd_state;
 
 #define ALBD_SCAN_BLOCKS 1024
@@ -128,6 +146,8 @@ ALBD_HIDDEN int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms); /* syn
 /* ---- fills (kernels, safe under graph capture) ---- */
 ALBD_HIDDEN int albd_fill_u32(albd_ctx *c, void *p, uint32_t v, size_t count);
 ALBD_HIDDEN int albd_fill_u64(albd_ctx *c, void *p, uint64_t v, size_t count);
+/* dst[i] = src[i] for count 64-bit words (16-byte aligned pointers) */
+ALBD_HIDDEN int albd_copy_u64(albd_ctx *c, void *dst, const void *src, size_t count);
 
 /* ---- BFS pieces ---- */
 /* haslen[w] from row_len */
@@ -158,6 +178,24 @@ ALBD_HIDDEN int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_st
 /* push: merge next into visited, write level/parent from pmin, build queue_next */
 ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                    int32_t level);
+/* Small-frontier push levels in one cooperative kernel of `blocks`
+ * workgroups (0: the default; capped at what can be co-resident): starting from the
+ * queue (st.queue, ctr->queue_len), run push levels level0, level0+1, ...
+ * while the new frontier has at most edge_cap out-edges and vertex_cap
+ * vertices, at most max_levels levels.
+ * On return the state is that of one finished push level: st.next holds the
+ * last frontier's bitmap, its queue is in st.queue_next (ctr->persist_in_queue
+ * = 0) or st.queue (= 1), ctr->qnext_len its length, the slots its counts;
+ * ctr->persist_levels levels ran, st.level_log holds their sizes.  A grid
+ * barrier that waits more than spin_limit polls sets ctr->persist_error and
+ * the kernel ends early (the keys in st.pkey must then be refilled). */
+/* albd_push_small's return when cooperative launches are unavailable on this
+ * context (another process may hold the device's cooperative resources); no
+ * kernel ran and the caller continues the level with the plain push */
+#define ALBD_NOCOOP 1
+ALBD_HIDDEN int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                                int32_t level0, int64_t edge_cap, int64_t vertex_cap,
+                                int32_t max_levels, int32_t blocks, uint32_t spin_limit);
 /* end of a level: sum the slot counters into ctr->frontier*, update the
  * running totals, compute the 12 features and the next level's choice
  * (MLP argmin when sel->params is set, else the reference heuristic) */

@@ -27,6 +27,7 @@ int albd_set_error_msg(int code, const char *msg)
 struct albd_ctx {
     int device;
     int ncu;
+    int coop_off;
     hipStream_t stream;
     hipEvent_t ev[ALBD_NEVENTS];
     albd_counters *hctr; // pinned
@@ -79,6 +80,8 @@ void albd_ctx_destroy(albd_ctx *c)
 int albd_ctx_device(const albd_ctx *c) { return c->device; }
 int albd_num_cus(const albd_ctx *c) { return c->ncu; }
 hipStream_t albd_stream(albd_ctx *c) { return c->stream; }
+int albd_coop_off(const albd_ctx *c) { return c->coop_off; }
+void albd_set_coop_off(albd_ctx *c) { c->coop_off = 1; }
 
 int albd_malloc(albd_ctx *c, size_t bytes, void **p)
 {
@@ -171,6 +174,16 @@ static __global__ __launch_bounds__(256) void k_fill_u64(uint64_t *p, uint64_t v
         p[i] = v;
 }
 
+// plain copy, 16 bytes per lane per step: the calibration kernel for the
+// device's achievable HBM bandwidth and for the memory traffic counters
+static __global__ __launch_bounds__(256) void k_copy_u64(ulonglong2 *dst, const ulonglong2 *src,
+                                                         size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 static unsigned fill_blocks(albd_ctx *c, size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -192,6 +205,18 @@ int albd_fill_u64(albd_ctx *c, void *p, uint64_t v, size_t count)
     if (!count) return 0;
     hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(c, count)), dim3(256), 0, c->stream,
                        (uint64_t *)p, v, count);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_copy_u64(albd_ctx *c, void *dst, const void *src, size_t count)
+{
+    if (!count) return 0;
+    if (count & 1) return albd_d2d(c, dst, src, count * 8);
+    const size_t n2 = count / 2;
+    size_t b = (n2 + 255) / 256, cap = (size_t)c->ncu * 32;
+    hipLaunchKernelGGL(k_copy_u64, dim3((unsigned)(b < 1 ? 1 : (b > cap ? cap : b))), dim3(256), 0,
+                       c->stream, (ulonglong2 *)dst, (const ulonglong2 *)src, n2);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -693,6 +693,315 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
     }
 }
 
+// ---------------------------------------------------------------------------
+// small-frontier push levels in one cooperative launch
+//
+// High-diameter graphs spend thousands of levels with a few thousand
+// frontier vertices; a launch sequence and a host round trip per level would
+// dominate, and one workgroup alone is bound by its CU's gather rate.  A
+// cooperative grid of G co-resident workgroups runs the push levels back to
+// back with one grid barrier per level, until the next frontier has more
+// than edge_cap out-edges or more than vertex_cap vertices.
+//
+// Exactness without a second barrier per level: every position p has a
+// 64-bit key, all ones outside this kernel.  An edge u -> v at level L does
+// atomicMin(key[v], L << 32 | id(u)).  The old value tells the discoverer
+// (all ones: first at this level), and the key keeps the smallest (level,
+// parent id) pair, i.e. the min-id parent at v's BFS level: keys of earlier
+// levels are smaller and never change.  The visited bitmap is only a filter
+// that saves atomics (a stale copy costs an atomic, never a wrong answer),
+// so level L's vertices are written out and merged into visited during
+// level L+1, and the keys go back to all ones when the kernel ends.
+//
+// Cross-workgroup data (found list, queues, keys, counters) moves through
+// device-coherent loads, stores and atomics; every wave waits for its own
+// memory operations before the barrier.
+
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallWaves = kSmallBlock / 64;
+
+template <class T> __device__ __forceinline__ T ld_coherent(const T *p)
+{
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// exclusive scan of one value per thread over the 1024-thread workgroup
+__device__ __forceinline__ int64_t small_block_scan(int64_t v, int64_t *total, int64_t *lds)
+{
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    int64_t wt;
+    const int64_t ex = wave_exclusive_scan<int64_t>(v, &wt);
+    if (lane == 0) lds[wid] = wt;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+    for (int i = 0; i < kSmallWaves; ++i) {
+        if (i < wid) off += lds[i];
+        tot += lds[i];
+    }
+    *total = tot;
+    return ex + off;
+}
+
+// append one entry per predicated lane to an LDS list: returns the slot of
+// this lane (valid where pred holds); works under any exec mask
+__device__ __forceinline__ unsigned int wave_append(unsigned int *counter, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0;
+    const int lane = lane_id();
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + (unsigned int)__popcll(m & lanemask_lt(lane));
+}
+
+constexpr int kSmallChunk = kSmallBlock;      // queue entries staged per pass
+constexpr int kSmallPer = 4;                  // edges per thread per window
+constexpr int kSmallWin = kSmallPer * kSmallBlock;
+
+template <class T> __device__ __forceinline__ void st_coherent(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// grid-wide barrier over a counter that only grows (reset by the host before
+// the launch): wave 0 of each workgroup arrives and waits until all G have
+// arrived at this generation.  Gives up after spin_limit polls or when
+// another workgroup gave up, so every wave always reaches the kernel's end.
+__device__ __forceinline__ bool grid_sync(albd_persist *P, unsigned int nblocks,
+                                          unsigned int spin_limit, int *s_ok)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned long long arrived = atomicAdd(&P->barrier, 1ull);
+        const unsigned long long target = (arrived / nblocks + 1) * nblocks;
+        unsigned int spins = 0;
+        while (ld_coherent(&P->barrier) < target) {
+            if (++spins > spin_limit || ld_coherent(&P->error) != 0) {
+                atomicOr(&P->error, 1u);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+struct SmallArgs {
+    albd_graph g;
+    albd_state st;
+    int32_t level0;
+    int32_t max_levels;
+    int64_t edge_cap;
+    int64_t vertex_cap;
+    uint32_t spin_limit;
+};
+
+__global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
+{
+    const albd_graph &g = a.g;
+    const albd_state &st = a.st;
+    __shared__ int64_t s_pre[kSmallChunk];
+    __shared__ int64_t s_start[kSmallChunk];
+    __shared__ uint32_t s_ou[kSmallChunk];
+    __shared__ int64_t s_scan[kSmallWaves];
+    __shared__ int32_t s_found[kSmallWin];
+    __shared__ int32_t s_queue[kSmallWin];
+    __shared__ unsigned int s_nf, s_nq;
+    __shared__ unsigned long long s_bf, s_bq, s_sum;
+    __shared__ int s_ok;
+    albd_persist *P = st.persist;
+    // cumulative list of the vertices this launch finds, level after level
+    int32_t *found = reinterpret_cast<int32_t *>(st.prefix);
+    int32_t *qa = st.queue, *qb = st.queue_next;
+    int64_t qlen = (int64_t)st.ctr->queue_len;
+    const int tid = threadIdx.x, lane = lane_id();
+    const unsigned int G = gridDim.x, w = blockIdx.x;
+    const int64_t gtid = (int64_t)w * kSmallBlock + tid, gthreads = (int64_t)G * kSmallBlock;
+    if (tid == 0) {
+        s_nf = 0;
+        s_nq = 0;
+        s_sum = 0;
+    }
+    __syncthreads();
+    int64_t fo_prev = 0, fo = 0; // found[fo_prev, fo): the previous level
+    bool ok = true;
+    for (int k = 0; ok; ++k) {
+        const uint32_t lvl = (uint32_t)(a.level0 + k + 1);
+        int64_t *cnt = st.level_log + 4 * k; // found, edges, queue entries of level k
+        // (1) the previous level: level, parent, visited
+        for (int64_t i = fo_prev + gtid; i < fo; i += gthreads) {
+            const int32_t v = ld_coherent(&found[i]);
+            const unsigned long long key = ld_coherent(&st.pkey[v]);
+            const int o = g.row_order[v];
+            st.level[o] = (int32_t)(key >> 32);
+            st.parent[o] = (int32_t)(uint32_t)key;
+            atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), 1ull << (v & 63));
+        }
+        // (2) expansion: workgroup w takes queue entries [q0, q1)
+        const int64_t per = (qlen + G - 1) / G;
+        const int64_t q0 = (int64_t)w * per, q1 = q0 + per < qlen ? q0 + per : qlen;
+        unsigned long long my_edges = 0;
+        for (int64_t base = q0; base < q1; base += kSmallChunk) { // uniform in the workgroup
+            const int cnt_e = (int)(q1 - base < kSmallChunk ? q1 - base : kSmallChunk);
+            int64_t a0 = 0, e0 = 0;
+            uint32_t o0 = 0;
+            if (tid < cnt_e) {
+                const int32_t u = ld_coherent(&qa[base + tid]);
+                a0 = g.push_ptr[u];
+                e0 = g.push_ptr[u + 1];
+                o0 = (uint32_t)g.row_order[u];
+            }
+            int64_t T;
+            const int64_t ex = small_block_scan(e0 - a0, &T, s_scan);
+            s_pre[tid] = ex;
+            s_start[tid] = a0;
+            s_ou[tid] = o0;
+            __syncthreads();
+            for (int64_t wb = 0; wb < T; wb += kSmallWin) { // uniform
+                int64_t e = wb + (int64_t)tid * kSmallPer;
+                const int64_t e_end = e + kSmallPer < T ? e + kSmallPer : T;
+                int lo = 0;
+                if (e < e_end) {
+                    int hi = cnt_e;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_pre[mid] <= e) lo = mid;
+                        else hi = mid;
+                    }
+                }
+                int32_t v[kSmallPer];
+                uint32_t ou[kSmallPer];
+                bool okv[kSmallPer];
+#pragma unroll
+                for (int i = 0; i < kSmallPer; ++i) {
+                    okv[i] = e < e_end;
+                    v[i] = 0;
+                    ou[i] = 0;
+                    if (okv[i]) {
+                        while (lo + 1 < cnt_e && s_pre[lo + 1] <= e) ++lo;
+                        v[i] = g.push_col[s_start[lo] + (e - s_pre[lo])];
+                        ou[i] = s_ou[lo];
+                        ++e;
+                    }
+                }
+                uint64_t vis[kSmallPer];
+#pragma unroll
+                for (int i = 0; i < kSmallPer; ++i) vis[i] = okv[i] ? st.visited[v[i] >> 6] : ~0ull;
+                bool won[kSmallPer];
+#pragma unroll
+                for (int i = 0; i < kSmallPer; ++i) {
+                    won[i] = false;
+                    if (!((vis[i] >> (v[i] & 63)) & 1ull)) {
+                        const unsigned long long key =
+                            ((unsigned long long)lvl << 32) | (unsigned long long)ou[i];
+                        const unsigned long long old = atomicMin(&st.pkey[v[i]], key);
+                        won[i] = old == ~0ull;
+                    }
+                }
+                int64_t dv[kSmallPer];
+#pragma unroll
+                for (int i = 0; i < kSmallPer; ++i)
+                    dv[i] = won[i] ? g.push_ptr[v[i] + 1] - g.push_ptr[v[i]] : 0;
+#pragma unroll
+                for (int i = 0; i < kSmallPer; ++i) {
+                    const unsigned int pf = wave_append(&s_nf, won[i]);
+                    if (won[i]) s_found[pf] = v[i];
+                    const unsigned int pq = wave_append(&s_nq, dv[i] > 0);
+                    if (dv[i] > 0) s_queue[pq] = v[i];
+                    my_edges += (unsigned long long)dv[i];
+                }
+                __syncthreads();
+                // flush the window's lists: one reservation per list
+                const unsigned int nf = s_nf, nq = s_nq;
+                if (tid == 0) {
+                    s_bf = nf ? atomicAdd(&P->found_total, (unsigned long long)nf) : 0;
+                    s_bq = nq ? atomicAdd(reinterpret_cast<unsigned long long *>(&cnt[2]),
+                                          (unsigned long long)nq)
+                              : 0;
+                }
+                __syncthreads();
+                const unsigned long long bf = s_bf, bq = s_bq;
+                for (unsigned int i = tid; i < nf; i += kSmallBlock) st_coherent(&found[bf + i], s_found[i]);
+                for (unsigned int i = tid; i < nq; i += kSmallBlock) st_coherent(&qb[bq + i], s_queue[i]);
+                __syncthreads();
+                if (tid == 0) {
+                    s_nf = 0;
+                    s_nq = 0;
+                }
+            }
+            __syncthreads(); // s_pre and friends are restaged
+        }
+        my_edges = wave_sum(my_edges);
+        if (lane == 0 && my_edges) atomicAdd(&s_sum, my_edges);
+        __syncthreads();
+        if (tid == 0) {
+            if (s_sum) atomicAdd(reinterpret_cast<unsigned long long *>(&cnt[1]), s_sum);
+            s_sum = 0;
+            // the next level's counters, unused until after the barrier
+            if (w == 0 && k + 1 < a.max_levels)
+                for (int f = 0; f < 4; ++f) st_coherent(&cnt[4 + f], (int64_t)0);
+        }
+        ok = grid_sync(P, G, a.spin_limit, &s_ok);
+        if (!ok) break;
+        const int64_t total = (int64_t)ld_coherent(&P->found_total);
+        const int64_t na = total - fo;
+        const int64_t edges = ld_coherent(&cnt[1]);
+        const int64_t nq = ld_coherent(&cnt[2]);
+        if (w == 0 && tid == 0) st_coherent(&cnt[0], na);
+        const bool last = na == 0 || edges > a.edge_cap || na > a.vertex_cap ||
+                          k + 1 >= a.max_levels;
+        if (last) {
+            // this level's vertices: level, parent, visited and next bits;
+            // then every key set by this launch goes back to all ones
+            unsigned long long my_words = 0;
+            for (int64_t i = fo + gtid; i < total; i += gthreads) {
+                const int32_t v = ld_coherent(&found[i]);
+                const unsigned long long key = ld_coherent(&st.pkey[v]);
+                const int o = g.row_order[v];
+                st.level[o] = (int32_t)(key >> 32);
+                st.parent[o] = (int32_t)(uint32_t)key;
+                const uint64_t bit = 1ull << (v & 63);
+                atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), bit);
+                const unsigned long long old =
+                    atomicOr(reinterpret_cast<unsigned long long *>(&st.next[v >> 6]), bit);
+                my_words += old == 0 ? 1 : 0;
+                st.pkey[v] = ~0ull;
+            }
+            for (int64_t i = gtid; i < fo; i += gthreads) st.pkey[ld_coherent(&found[i])] = ~0ull;
+            my_words = wave_sum(my_words);
+            if (lane == 0 && my_words) atomicAdd(&st.slots[1 + (w % (ALBD_NSLOT - 1))].next_words, my_words);
+            if (w == 0 && tid == 0) {
+                st.slots[0].next_count = (unsigned long long)na;
+                st.slots[0].next_edges = (unsigned long long)edges;
+                st.ctr->qnext_len = (unsigned long long)nq;
+                st.ctr->persist_levels = k + 1;
+                st.ctr->persist_in_queue = qb == st.queue ? 1 : 0;
+            }
+            break;
+        }
+        // an intermediate level: the select kernel sees only the last one,
+        // so these counts go straight into the running totals
+        if (w == 0 && tid == 0) {
+            st.ctr->reached += (unsigned long long)na;
+            st.ctr->reached_edges += (unsigned long long)edges;
+        }
+        fo_prev = fo;
+        fo = total;
+        int32_t *t = qa;
+        qa = qb;
+        qb = t;
+        qlen = nq;
+    }
+    if (!ok && tid == 0) st.ctr->persist_error = 1;
+}
+
 int grid_persistent(albd_ctx *c, int blocks_per_cu)
 {
     int g = albd_num_cus(c) * blocks_per_cu;
@@ -865,6 +1174,36 @@ int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
     hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, albd_stream(c), *g, *st);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level0,
+                    int64_t edge_cap, int64_t vertex_cap, int32_t max_levels, int32_t blocks,
+                    uint32_t spin_limit)
+{
+    if (max_levels > ALBD_PERSIST_LEVELS) max_levels = ALBD_PERSIST_LEVELS;
+    if (max_levels < 1) max_levels = 1;
+    if (albd_coop_off(c)) return ALBD_NOCOOP;
+    // every workgroup of the grid barrier must be resident at once
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
+    const int cap = per_cu * albd_num_cus(c);
+    if (blocks <= 0) blocks = 64;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) return albd_set_error_msg(ALBBA_EHIP, "small-frontier kernel cannot be resident");
+    hipStream_t s = albd_stream(c);
+    // level 0's counters, the grid state and the error flag start at zero
+    if (int rc = albd_fill_u64(c, st->level_log, 0, 4)) return rc;
+    if (int rc = albd_fill_u64(c, st->persist, 0, sizeof(albd_persist) / 8)) return rc;
+    if (int rc = albd_fill_u32(c, &st->ctr->persist_error, 0, 1)) return rc;
+    SmallArgs a{*g, *st, level0, max_levels, edge_cap, vertex_cap, spin_limit};
+    void *args[] = {&a};
+    if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_push_small), dim3(blocks),
+                                   dim3(kSmallBlock), args, 0, s) != hipSuccess) {
+        (void)hipGetLastError(); // nothing was launched; clear the error state
+        albd_set_coop_off(c);
+        return ALBD_NOCOOP;
+    }
     return 0;
 }
 

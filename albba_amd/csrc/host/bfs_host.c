@@ -37,6 +37,7 @@ struct albba_graph {
     int64_t device_bytes;
     void *scratch_visited; /* BEST-policy snapshots */
     void *scratch_ctr;
+    int64_t *host_log;     /* small-frontier level sizes, 4 per level */
 };
 
 /* ---- context ---- */
@@ -79,6 +80,31 @@ int albba_ctx_synchronize(albba_ctx *c)
 }
 
 albd_ctx *albba_ctx_dev(albba_ctx *c) { return c ? c->d : NULL; }
+
+int albba_bench_copy(albba_ctx *c, int64_t bytes, int reps, float *ms)
+{
+    if (!c || !ms || bytes < 16 || reps < 1) return albba_fail(ALBBA_EINVAL, "bench_copy: bad argument");
+    const size_t words = (size_t)(bytes / 16) * 2;
+    void *a = NULL, *b = NULL;
+    int rc = albd_malloc(c->d, words * 8, &a);
+    if (!rc) rc = albd_malloc(c->d, words * 8, &b);
+    /* one untimed copy touches both buffers first */
+    if (!rc) rc = albd_fill_u64(c->d, a, 0x5a5a5a5a5a5a5a5aull, words);
+    if (!rc) rc = albd_copy_u64(c->d, b, a, words);
+    if (!rc) rc = albd_event_record(c->d, 4);
+    for (int r = 0; !rc && r < reps; ++r) rc = albd_copy_u64(c->d, b, a, words);
+    if (!rc) rc = albd_event_record(c->d, 5);
+    float t = 0;
+    if (!rc) rc = albd_event_elapsed(c->d, 4, 5, &t);
+    if (!rc) rc = albd_sync(c->d);
+    char buf[256] = "";
+    if (rc) snprintf(buf, sizeof buf, "%s", albd_error());
+    albd_free(c->d, a);
+    albd_free(c->d, b);
+    if (rc) return albba_fail(rc, "bench_copy: %s", buf);
+    *ms = t / (float)reps;
+    return ALBBA_OK;
+}
 
 /* ---- upload ---- */
 
@@ -148,9 +174,13 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, G->st.active_next);
     albd_free(d, G->st.ctr);
     albd_free(d, G->st.slots);
+    albd_free(d, G->st.level_log);
+    albd_free(d, G->st.pkey);
+    albd_free(d, G->st.persist);
     albd_free(d, G->scratch_visited);
     albd_free(d, G->scratch_ctr);
     free(G->pos_of);
+    free(G->host_log);
     free(G);
 }
 
@@ -212,6 +242,9 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     st->active_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
     st->ctr = (albd_counters *)dalloc(G, sizeof(albd_counters), &rc);
     st->slots = (albd_slot *)dalloc(G, sizeof(albd_slot) * ALBD_NSLOT, &rc);
+    st->level_log = (int64_t *)dalloc(G, sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS, &rc);
+    st->pkey = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
+    st->persist = (albd_persist *)dalloc(G, sizeof(albd_persist), &rc);
     G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
     if (rc) {
@@ -235,6 +268,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         (s->pull_ptr && (rc = upload_padded_i64(ctx, (int64_t *)g->pull_ptr, s->pull_ptr, npos_raw + 1, npos + 1))) ||
         (s->pull_ptr && (rc = albd_h2d(ctx->d, (void *)g->pull_col, s->pull_col, sizeof(int32_t) * (size_t)s->nnz))) ||
         (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
+        (rc = albd_fill_u64(ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
         (rc = albd_sync(ctx->d))) {
         char buf[512];
         snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "host staging buffer" : albd_error());
@@ -242,7 +276,8 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         return albba_fail(rc, "graph_upload: %s", buf);
     }
     G->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)s->n);
-    if (!G->pos_of) { albba_graph_free(G); return albba_fail(ALBBA_ENOMEM, "graph_upload: pos_of"); }
+    G->host_log = (int64_t *)albba_malloc(sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS);
+    if (!G->pos_of || !G->host_log) { albba_graph_free(G); return albba_fail(ALBBA_ENOMEM, "graph_upload: host arrays"); }
     memcpy(G->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n);
     *out = G;
     return ALBBA_OK;
@@ -259,6 +294,7 @@ void albba_bfs_opts_default(albba_bfs_opts *o)
     o->switch_policy = ALBBA_SWITCH_HEURISTIC;
     o->fixed_choice = ALBBA_CHOICE_PULL_SB;
     o->timing_reps = 3;
+    o->small_frontier_edges = 65536;
 }
 
 enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4 };
@@ -324,6 +360,54 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
     const int64_t bound = ls->list_valid ? ls->list_len : G->g.nwords;
     if ((rc = albd_pull(d, &G->g, &G->st, variant, level, bound))) return rc;
     return mark ? albd_event_record(d, 7) : 0;
+}
+
+/* Push levels in the single-workgroup kernel, starting with the level whose
+ * frontier is st.queue (or the front bitmap after a pull).  Returns the
+ * number of levels run in *nlev; afterwards the state is that of one
+ * finished push level (queue_next holds the last frontier's queue). */
+static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_t frontier,
+                     int64_t edge_cap, int64_t vertex_cap, int32_t blocks, int mark, int *nlev)
+{
+    albd_ctx *d = G->ctx->d;
+    int flags = 0;
+    if (ls->prev_push) flags |= LB_QUEUE_FROM_NEXT;
+    if (ls->list_pending) flags |= LB_ACTIVE_FROM_NEXT;
+    if (!ls->have_queue) flags |= LB_ZERO_QUEUE;
+    int rc = albd_level_begin_flags(d, &G->g, &G->st, flags);
+    if (rc) return rc;
+    if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
+    if (mark && (rc = albd_event_record(d, 6))) return rc;
+    rc = albd_push_small(d, &G->g, &G->st, level, edge_cap, vertex_cap, ALBD_PERSIST_LEVELS,
+                         blocks, 1u << 22);
+    if (rc == ALBD_NOCOOP) {
+        /* no cooperative launch on this device right now: the level runs as
+         * a plain push from the queue already built */
+        *nlev = 1;
+        if ((rc = albd_push_scan(d, &G->g, &G->st, frontier)) ||
+            (rc = albd_push_expand(d, &G->g, &G->st)) || (mark && (rc = albd_event_record(d, 7))))
+            return rc;
+        return albd_push_finalize(d, &G->g, &G->st, level);
+    }
+    if (rc) return rc;
+    if (mark && (rc = albd_event_record(d, 7))) return rc;
+    if ((rc = albd_read_counters(d, &G->st))) return rc;
+    const albd_counters *hc = albd_host_counters(d);
+    if (hc->persist_error) {
+        /* a grid barrier gave up: the keys may be left set */
+        albd_fill_u64(d, G->st.pkey, ~0ull, (size_t)G->g.npos);
+        albd_sync(d);
+        return albba_fail(ALBBA_EHIP, "small-frontier kernel: grid barrier timed out");
+    }
+    *nlev = hc->persist_levels;
+    if (hc->persist_in_queue) {
+        /* the last frontier's queue is in st.queue: swap so that it is in
+         * queue_next, where a finished push level leaves it */
+        int32_t *q = G->st.queue;
+        G->st.queue = G->st.queue_next;
+        G->st.queue_next = q;
+    }
+    return 0;
 }
 
 /* Pointer swaps and bookkeeping after a committed level. */
@@ -392,6 +476,8 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
     albd_select_args sel = {policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(opts.selector) : NULL,
                        0, ALBBA_CHOICE_PUSH, valid_mask, 1};
 
+    /* the heuristic pushes while F <= (int)(n * 0.04) */
+    const int64_t push_vertex_cap = (int64_t)(int)((double)G->n * 0.04);
     loop_state ls = {1, 0, 0, 0, 0};
     int32_t level = 0;
     int rc;
@@ -442,11 +528,28 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
                 (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))))
                 return albba_fail(rc, "bfs: %s", albd_error());
         }
+        /* a pushed level with a small frontier starts a run of push levels in
+         * the single-workgroup kernel; it continues while the frontier stays
+         * within the edge cap and within the heuristic's push threshold
+         * (bfs/src/bfs_SELLCSv2.c:282-284), so the heuristic's choices are
+         * unchanged */
+        const int64_t first_edges = (int64_t)hc->frontier_edges;
+        const int small = choice == ALBBA_CHOICE_PUSH && opts.small_frontier_edges > 0 &&
+                          (policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED) &&
+                          first_edges <= opts.small_frontier_edges && frontier <= push_vertex_cap;
+        int nlev = 1;
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
-        if ((rc = run_level(G, &ls, choice, level, frontier, opts.time_levels)))
-            return albba_fail(rc, "bfs: %s", albd_error());
+        if (small)
+            rc = run_small(G, &ls, level, frontier, opts.small_frontier_edges, push_vertex_cap,
+                           opts.small_frontier_blocks, opts.time_levels, &nlev);
+        else
+            rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
+        if (rc) return albba_fail(rc, "bfs: %s", albd_error());
         if (opts.time_levels && (rc = albd_event_record(d, 3))) return albba_fail(rc, "bfs: %s", albd_error());
-        sel.level = level + 1;
+        if (small && nlev > 1 && stats &&
+            (rc = albd_d2h(d, G->host_log, G->st.level_log, sizeof(int64_t) * 4 * (size_t)(nlev - 1))))
+            return albba_fail(rc, "bfs: %s", albd_error());
+        sel.level = level + nlev;
         sel.prev_choice = choice;
         sel.first = 0;
         if ((rc = albd_select(d, &G->g, &G->st, &sel)) || (rc = albd_read_counters(d, &G->st)))
@@ -454,28 +557,40 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
         level_sums sums;
         sum_counters(hc, &sums);
         if (stats && level < stats->capacity) {
-            if (stats->frontier) stats->frontier[level] = (int32_t)frontier;
-            if (stats->choice) stats->choice[level] = choice;
-            if (stats->features) memcpy(stats->features + (size_t)level * ALBBA_NUM_FEATURES, feats, sizeof feats);
-            if (stats->choice_ms) memcpy(stats->choice_ms + (size_t)level * ALBBA_NUM_CHOICES, choice_ms, sizeof choice_ms);
-            if (stats->level_ms) {
-                float ms = 0;
-                if (opts.time_levels) albd_event_elapsed(d, 2, 3, &ms);
-                stats->level_ms[level] = ms;
+            float lms = 0, kms = 0;
+            if (opts.time_levels) {
+                albd_event_elapsed(d, 2, 3, &lms);
+                albd_event_elapsed(d, 6, 7, &kms);
             }
-            if (stats->kernel_ms) {
-                float ms = 0;
-                if (opts.time_levels) albd_event_elapsed(d, 6, 7, &ms);
-                stats->kernel_ms[level] = ms;
+            for (int j = 0; j < nlev && level + j < stats->capacity; ++j) {
+                const int32_t lv = level + j;
+                const int push = choice == ALBBA_CHOICE_PUSH;
+                /* levels after the first of a small-frontier run take their
+                 * sizes from the kernel's level log */
+                const int64_t fj = j == 0 ? frontier : G->host_log[4 * (j - 1)];
+                if (stats->frontier) stats->frontier[lv] = (int32_t)fj;
+                if (stats->choice) stats->choice[lv] = choice;
+                if (stats->features) {
+                    float *f = stats->features + (size_t)lv * ALBBA_NUM_FEATURES;
+                    if (j == 0) memcpy(f, feats, sizeof feats);
+                    else memset(f, 0, sizeof feats);
+                }
+                if (stats->choice_ms) memcpy(stats->choice_ms + (size_t)lv * ALBBA_NUM_CHOICES, choice_ms, sizeof choice_ms);
+                if (stats->level_ms) stats->level_ms[lv] = lms / (float)nlev;
+                if (stats->kernel_ms) stats->kernel_ms[lv] = kms / (float)nlev;
+                if (small) {
+                    if (stats->entries) stats->entries[lv] = j == 0 ? first_edges : G->host_log[4 * (j - 1) + 1];
+                    if (stats->words) stats->words[lv] = fj;
+                } else {
+                    if (stats->entries) stats->entries[lv] = push ? sums.push_edges : sums.scanned;
+                    if (stats->words) stats->words[lv] = push ? sums.queue_len : sums.words;
+                }
+                if (stats->bypassed) stats->bypassed[lv] = push ? 0 : sums.bypassed;
             }
-            const int push = choice == ALBBA_CHOICE_PUSH;
-            if (stats->entries) stats->entries[level] = push ? sums.push_edges : sums.scanned;
-            if (stats->words) stats->words[level] = push ? sums.queue_len : sums.words;
-            if (stats->bypassed) stats->bypassed[level] = push ? 0 : sums.bypassed;
         }
         commit_level(G, &ls, choice, &sums);
         frontier = (int64_t)hc->frontier;
-        ++level;
+        level += nlev;
     }
     float ms = 0;
     if ((rc = albd_event_record(d, 1)) || (rc = albd_event_elapsed(d, 0, 1, &ms)))

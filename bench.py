@@ -41,31 +41,63 @@ def push_bytes(queue, edges, n, n_new):
     return 4 * queue + 8 * queue + 4 * edges + 8 * n_new + n / 8
 
 
-def run_policy(dg, roots, policy, sel, steps, nnz, n, prof):
-    """Time `steps` passes over all roots; returns per-root mean ms, step ms."""
-    import albba_amd as A
+# kernel (as rocprofv3 names it) that does the main work of each choice
+KERNEL_OF_CHOICE = {0: "k_push_expand", 1: "k_pull<64, 0>", 2: "k_pull<64, 1>",
+                    3: "k_pull<64, 2>"}
 
+
+def run_policy(dg, roots, policy, sel, steps):
+    """Time `steps` passes over all roots (no per-level events); returns the
+    per-root mean BFS ms (events around each BFS) and the mean step ms."""
     per_root = np.zeros(len(roots))
     step_ms = []
     for s in range(steps):
+        dg.ctx.synchronize()
         t0 = time.perf_counter()
         for i, r in enumerate(roots):
-            o = dg.bfs(int(r), policy=policy, selector=sel, time_levels=True, results=False)
+            o = dg.bfs(int(r), policy=policy, selector=sel, results=False)
             per_root[i] += o["bfs_ms"] / steps
-            L = o["num_levels"]
-            for lv in range(L):
-                n_new = o["frontier"][lv + 1] if lv + 1 < L else 0
-                kind = "push" if o["choice"][lv] == A.CHOICE_PUSH else "pull"
-                b = (push_bytes(o["words"][lv], o["entries"][lv], n, n_new) if kind == "push"
-                     else pull_bytes(o["entries"][lv], o["words"][lv], n, n_new))
-                p = prof.setdefault(kind, {"ms": 0.0, "bytes": 0.0, "launches": 0})
-                p["ms"] += float(o["kernel_ms"][lv])
-                p["bytes"] += float(b)
-                p["launches"] += 1
         dg.ctx.synchronize()
         step_ms.append((time.perf_counter() - t0) * 1e3)
         log(f"  step {s}: {step_ms[-1]:.1f} ms")
     return per_root, float(np.mean(step_ms))
+
+
+def account_kernels(dg, roots, policy, sel, n):
+    """One pass with HIP events around every level's main kernel (on the
+    library's stream): algorithmic bytes and time per kernel."""
+    import albba_amd as A
+
+    prof = {}
+    for r in roots:
+        o = dg.bfs(int(r), policy=policy, selector=sel, time_levels=True, results=False)
+        L = o["num_levels"]
+        for lv in range(L):
+            n_new = o["frontier"][lv + 1] if lv + 1 < L else 0
+            ch = int(o["choice"][lv])
+            b = (push_bytes(o["words"][lv], o["entries"][lv], n, n_new) if ch == A.CHOICE_PUSH
+                 else pull_bytes(o["entries"][lv], o["words"][lv], n, n_new))
+            p = prof.setdefault(KERNEL_OF_CHOICE[ch], {"ms": 0.0, "bytes": 0.0, "launches": 0})
+            p["ms"] += float(o["kernel_ms"][lv])
+            p["bytes"] += float(b)
+            p["launches"] += 1
+    return prof
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "k_pull_rmat_s24_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split("|")
+        if len(parts) > 3 and parts[1].strip() == kernel:
+            try:
+                return float(parts[-2])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(g, roots, gpu_levels):
@@ -111,6 +143,8 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--roots", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--policy", choices=["both", "heuristic", "learned"], default="both",
+                    help="run one policy only (profiling); 'both' checks bit-exactness")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -139,6 +173,11 @@ def main():
     policies = [("heuristic", A.SWITCH_HEURISTIC, None)]
     if sel is not None:
         policies.append(("learned", A.SWITCH_LEARNED, sel))
+    if args.policy != "both":
+        policies = [p for p in policies if p[0] == args.policy]
+        if not policies:
+            log(f"policy {args.policy} unavailable (no selector file)")
+            sys.exit(1)
 
     # warmup + bit-exactness: every policy must give identical level/parent
     bit_exact = True
@@ -156,32 +195,42 @@ def main():
                     else:
                         bit_exact &= bool(np.array_equal(ref[0], o["level"]) and
                                           np.array_equal(ref[1], o["parent"]))
+    if len(policies) < 2:
+        bit_exact = None  # nothing to compare against
     log(f"warmup done, policies bit-exact: {bit_exact}")
 
     results = {}
     for name, pol, s in policies:
-        prof = {}
         log(f"timing policy {name}")
-        per_root, step_ms = run_policy(dg, roots, pol, s, args.steps, g.nnz, g.n, prof)
+        per_root, step_ms = run_policy(dg, roots, pol, s, args.steps)
         gt = [A.gteps(g.nnz, ms) for ms in per_root]
         results[name] = {"hm_gteps_events": A.harmonic_mean(gt), "step_ms": step_ms,
-                         "wall_gteps": len(roots) * g.nnz / (step_ms * 1e6), "prof": prof,
+                         "wall_gteps": len(roots) * g.nnz / (step_ms * 1e6),
                          "mean_bfs_ms": float(np.mean(per_root))}
         log(f"  {name}: harmonic-mean GTEPS (events) {results[name]['hm_gteps_events']:.2f}, "
             f"wall {results[name]['wall_gteps']:.2f}, step {step_ms:.1f} ms")
 
     main_name = "learned" if "learned" in results else "heuristic"
     R = results[main_name]
-    prof = R["prof"]
+    pol_main, sel_main = [(p, s) for nm, p, s in policies if nm == main_name][0]
+    prof = account_kernels(dg, roots, pol_main, sel_main, g.n)
     dom = max(prof, key=lambda k: prof[k]["ms"])
     P = prof[dom]
     achieved = P["bytes"] / (P["ms"] * 1e-3) if P["ms"] > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": "k_pull" if dom == "pull" else "k_push_expand",
+    copy_bytes = 1 << 30
+    copy_ms = ctx.bench_copy(copy_bytes, 20)
+    roofline = {"bound": "hbm", "kernel": dom,
                 "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profile(dom),
                 "avg_launch_ms": round(P["ms"] / max(P["launches"], 1), 5),
                 "avg_bytes_per_launch": round(P["bytes"] / max(P["launches"], 1)),
-                "launches": P["launches"]}
+                "launches": P["launches"],
+                "copy_kernel_gbps": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
+                "kernels": {k: {"launches": v["launches"],
+                                "avg_ms": round(v["ms"] / max(v["launches"], 1), 5),
+                                "gbps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
+                                if v["ms"] > 0 else None}
+                            for k, v in sorted(prof.items())}}
     line = {
         "metric": METRIC, "value": round(R["wall_gteps"], 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(R["step_ms"], 3),
@@ -192,10 +241,11 @@ def main():
                    "scale": args.scale, "edgefactor": 16, "a": 0.57, "b": 0.19, "c": 0.19,
                    "n": g.n, "nnz_stored": g.nnz, "roots": len(roots), "root_seed": 7,
                    "layout": "SELL-C-sigma C=64 sigma=4096 pad=1", "policy": main_name},
-        "heuristic_gteps": round(results["heuristic"]["wall_gteps"], 3),
+        "heuristic_gteps": (round(results["heuristic"]["wall_gteps"], 3)
+                            if "heuristic" in results else None),
         "learned_over_heuristic": (round(results["learned"]["wall_gteps"] /
                                          results["heuristic"]["wall_gteps"], 4)
-                                   if "learned" in results else None),
+                                   if len(results) == 2 else None),
         "hm_gteps_events": {k: round(v["hm_gteps_events"], 3) for k, v in results.items()},
         "bit_exact": bit_exact,
         "roofline": roofline,

@@ -213,6 +213,15 @@ typedef struct albba_bfs_opts {
     const albba_selector *selector; /* for ALBBA_SWITCH_LEARNED                    */
     int32_t time_levels;         /* 1: record per-level kernel times (events)      */
     int32_t timing_reps;         /* ALBBA_SWITCH_BEST / sampling: reps per choice  */
+    /* Heuristic and learned policies: when a level pushes and its frontier has
+     * at most this many out-edges, consecutive levels run inside one
+     * cooperative push kernel (one grid barrier per level, no host round
+     * trip) until the frontier outgrows the cap or the heuristic's push
+     * threshold (0 disables; default 65536).  Results are identical either
+     * way. */
+    int64_t small_frontier_edges;
+    int32_t small_frontier_blocks; /* workgroups of that kernel (0: default)    */
+    int32_t reserved_;
 } albba_bfs_opts;
 
 /* Per-level record; arrays are caller-owned with `capacity` entries. */
@@ -282,6 +291,13 @@ ALBBA_API int albba_ctx_synchronize(albba_ctx *ctx);
 ALBBA_API int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out);
 ALBBA_API void albba_graph_free(albba_graph *g);
 ALBBA_API int64_t albba_graph_device_bytes(const albba_graph *g);
+
+/* Device copy bandwidth: copies `bytes` (rounded down to 16) between two
+ * fresh device buffers `reps` times on the context's stream and returns the
+ * mean time of one copy in *ms.  No counterpart in the reference: it gives
+ * the achievable HBM bandwidth next to the BFS kernels' figures and is the
+ * known-size kernel that memory traffic counters are scaled by. */
+ALBBA_API int albba_bench_copy(albba_ctx *ctx, int64_t bytes, int reps, float *ms);
 
 ALBBA_API void albba_bfs_opts_default(albba_bfs_opts *o);
 /* BFS from vertex `root` (original id).  Results stay on the device until

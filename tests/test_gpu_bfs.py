@@ -199,3 +199,26 @@ def test_cli_driver_replays_the_reference_sequence(tmp_path):
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.count("tree ok") == 4
+
+
+@pytest.mark.parametrize("cap,blocks", [(0, 0), (1, 0), (8, 3), (64, 1), (1024, 7), (65536, 0),
+                                        (65536, 1), (65536, 64)])
+def test_small_frontier_push_runs(ctx, cap, blocks):
+    """Runs of push levels inside the cooperative small-frontier kernel: edge
+    caps that end a run after every level, after a few, or never, grids of 1
+    to 64 workgroups, and the kernel switched off (cap 0) all give the
+    oracle's levels, parents and per-level frontier sizes.  The RMAT graph
+    switches to pull in the middle, so runs also start from a pulled
+    frontier bitmap; repeated BFS on one device graph check that the kernel
+    leaves its per-vertex keys reset."""
+    for g in (A.Graph.grid(120, 90, 0.1, seed=51, perm_seed=52),
+              A.Graph.rmat(13, 8, seed=53, perm_seed=54),
+              A.Graph.er(3000, 3.0, seed=55, perm_seed=56)):
+        rp, col = g.row_ptr.copy(), g.col.copy()
+        dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+        for root in g.pick_roots(3, 57, largest_cc=True):
+            o = check(dg, rp, col, root, small_frontier_edges=cap, small_frontier_blocks=blocks,
+                      capacity=1 << 12)
+            assert (o["choice"][o["frontier"] > 0] >= 0).all()
+            check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PUSH, small_frontier_edges=cap)
+        dg.close()
