@@ -146,7 +146,9 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
                         else
                             act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
                         av[u] = mine && col < len;
-                        if (av[u]) cidx[u] = g.colidx[cp + l];
+                        // streaming: non-temporal, so that the frontier and
+                        // visited bitmaps stay in L2
+                        if (av[u]) cidx[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
                         cp += (act + padm) & ~padm;
                     }
                 }
@@ -232,9 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
                     a.st.visited[w] = vis | found;
                 }
                 if ((found >> lane) & 1ull) {
-                    const int o = g.row_order[p];
-                    a.st.level[o] = a.level + 1;
-                    a.st.parent[o] = g.row_order[parent_pos];
+                    // position order: the lanes of a word write one run
+                    a.st.level_pos[p] = a.level + 1;
+                    a.st.parent_pos[p] = g.row_order[parent_pos];
                     edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
                 }
                 cnt += __popcll(found);
@@ -262,14 +264,12 @@ __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint6
 __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, int64_t npos,
                                                       int32_t nwords)
 {
-    const int64_t total = npos > n ? npos : n;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+    (void)n;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npos;
          i += (int64_t)gridDim.x * kBlock) {
-        if (i < n) {
-            st.level[i] = -1;
-            st.parent[i] = -1;
-        }
-        if (i < npos) st.pmin[i] = 0xFFFFFFFFu;
+        st.level_pos[i] = -1;
+        st.parent_pos[i] = -1;
+        st.pmin[i] = 0xFFFFFFFFu;
         if (i < nwords) {
             st.visited[i] = 0;
             st.front[i] = 0;
@@ -288,8 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, 
 
 __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 {
-    st.level[ro] = 0;
-    st.parent[ro] = ro;
+    st.level_pos[rp] = 0;
+    st.parent_pos[rp] = ro;
     const uint64_t bit = 1ull << (rp & 63);
     st.visited[rp >> 6] = bit;
     st.front[rp >> 6] = bit;
@@ -502,9 +502,8 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
                 const int64_t d = g.push_ptr[pp + 1] - g.push_ptr[pp];
                 keep = d > 0;
                 if (FINALIZE) {
-                    const int o = g.row_order[pp];
-                    st.level[o] = level + 1;
-                    st.parent[o] = (int32_t)st.pmin[pp];
+                    st.level_pos[pp] = level + 1;
+                    st.parent_pos[pp] = (int32_t)st.pmin[pp];
                     edges += (unsigned long long)d;
                 }
             }
@@ -533,6 +532,18 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
         nzw = wave_sum(nzw);
         edges = wave_sum(edges);
         if (lane == 0) slot_add(st.slots, cnt, edges, nzw, 0);
+    }
+}
+
+// level and parent from position order to vertex order, once per BFS: the
+// writes are coalesced, the reads gather within a sigma window
+__global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < g.n;
+         v += (int64_t)gridDim.x * kBlock) {
+        const int32_t p = g.pos_of[v];
+        st.level[v] = st.level_pos[p];
+        st.parent[v] = st.parent_pos[p];
     }
 }
 
@@ -839,9 +850,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
         for (int64_t i = fo_prev + gtid; i < fo; i += gthreads) {
             const int32_t v = ld_coherent(&found[i]);
             const unsigned long long key = ld_coherent(&st.pkey[v]);
-            const int o = g.row_order[v];
-            st.level[o] = (int32_t)(key >> 32);
-            st.parent[o] = (int32_t)(uint32_t)key;
+            st.level_pos[v] = (int32_t)(key >> 32);
+            st.parent_pos[v] = (int32_t)(uint32_t)key;
             atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), 1ull << (v & 63));
         }
         // (2) expansion: workgroup w takes queue entries [q0, q1)
@@ -964,9 +974,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
             for (int64_t i = fo + gtid; i < total; i += gthreads) {
                 const int32_t v = ld_coherent(&found[i]);
                 const unsigned long long key = ld_coherent(&st.pkey[v]);
-                const int o = g.row_order[v];
-                st.level[o] = (int32_t)(key >> 32);
-                st.parent[o] = (int32_t)(uint32_t)key;
+                st.level_pos[v] = (int32_t)(key >> 32);
+                st.parent_pos[v] = (int32_t)(uint32_t)key;
                 const uint64_t bit = 1ull << (v & 63);
                 atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), bit);
                 const unsigned long long old =
@@ -1068,6 +1077,16 @@ int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_
                        *st, g->n, g->npos, g->nwords);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, *g, *st, rp, ro);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st)
+{
+    int64_t blocks = ((int64_t)g->n + kBlock - 1) / kBlock;
+    int64_t cap = (int64_t)albd_num_cus(c) * 16;
+    hipLaunchKernelGGL(k_to_vertex_order, dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))),
+                       dim3(kBlock), 0, albd_stream(c), *g, *st);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -159,6 +159,9 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, (void *)G->g.push_ptr);
     albd_free(d, (void *)G->g.push_col);
     albd_free(d, (void *)G->g.haslen);
+    albd_free(d, (void *)G->g.pos_of);
+    albd_free(d, G->st.level_pos);
+    albd_free(d, G->st.parent_pos);
     albd_free(d, G->st.visited);
     albd_free(d, G->st.front);
     albd_free(d, G->st.next);
@@ -225,6 +228,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         g->pull_col = g->push_col;
     }
     g->haslen = (const uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    g->pos_of = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->n > 0 ? s->n : 1), &rc);
     albd_state *st = &G->st;
     const int64_t nsum = (nwords + 63) / 64 + 1;
     st->visited = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
@@ -236,6 +240,8 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     st->prefix = (int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
     st->block_sums = (int64_t *)dalloc(G, sizeof(int64_t) * (ALBD_SCAN_BLOCKS + 1), &rc);
     st->pmin = (uint32_t *)dalloc(G, sizeof(uint32_t) * (size_t)npos, &rc);
+    st->level_pos = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->parent_pos = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
     st->level = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
     st->parent = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
     st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
@@ -267,6 +273,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         (rc = albd_h2d(ctx->d, (void *)g->push_col, s->push_col, sizeof(int32_t) * (size_t)s->nnz)) ||
         (s->pull_ptr && (rc = upload_padded_i64(ctx, (int64_t *)g->pull_ptr, s->pull_ptr, npos_raw + 1, npos + 1))) ||
         (s->pull_ptr && (rc = albd_h2d(ctx->d, (void *)g->pull_col, s->pull_col, sizeof(int32_t) * (size_t)s->nnz))) ||
+        (rc = albd_h2d(ctx->d, (void *)g->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n)) ||
         (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
         (rc = albd_fill_u64(ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
         (rc = albd_sync(ctx->d))) {
@@ -593,7 +600,8 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
         level += nlev;
     }
     float ms = 0;
-    if ((rc = albd_event_record(d, 1)) || (rc = albd_event_elapsed(d, 0, 1, &ms)))
+    if ((rc = albd_to_vertex_order(d, &G->g, &G->st)) || (rc = albd_event_record(d, 1)) ||
+        (rc = albd_event_elapsed(d, 0, 1, &ms)))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (stats) {
         stats->num_levels = level;
