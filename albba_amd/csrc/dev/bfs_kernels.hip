@@ -251,6 +251,30 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
 // ---------------------------------------------------------------------------
 // word passes
 
+// One reservation per block in a global list: every thread calls it with its
+// wave's count (lane 0's value counts); returns the wave's first slot.  Same
+// address device atomics serialise, so the word passes take one per block
+// step rather than one per wave.
+__device__ __forceinline__ unsigned long long block_reserve(unsigned long long *counter,
+                                                           unsigned int wave_count,
+                                                           unsigned int *s_cnt,
+                                                           unsigned long long *s_base)
+{
+    const int wid = threadIdx.x >> 6;
+    if (lane_id() == 0) s_cnt[wid] = wave_count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int tot = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) tot += s_cnt[i];
+        *s_base = tot ? atomicAdd(counter, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long off = *s_base;
+    for (int i = 0; i < wid; ++i) off += s_cnt[i];
+    __syncthreads(); // s_cnt is reused by the next call
+    return off;
+}
+
 __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint64_t *haslen,
                                                    int32_t nwords)
 {
@@ -264,17 +288,16 @@ __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint6
 __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, int64_t npos,
                                                       int32_t nwords)
 {
+    // level_pos/parent_pos hold stale values outside the visited set (the
+    // vertex-order pass masks them), and pmin is all ones at rest (the push
+    // finalize restores every entry it consumed), so only the bitmaps reset
     (void)n;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npos;
+    (void)npos;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords;
          i += (int64_t)gridDim.x * kBlock) {
-        st.level_pos[i] = -1;
-        st.parent_pos[i] = -1;
-        st.pmin[i] = 0xFFFFFFFFu;
-        if (i < nwords) {
-            st.visited[i] = 0;
-            st.front[i] = 0;
-            st.next[i] = 0;
-        }
+        st.visited[i] = 0;
+        st.front[i] = 0;
+        st.next[i] = 0;
     }
     unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -308,19 +331,22 @@ __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 }
 
 // End of a level: the frontier it produced, the running totals, the
-// selector features and the policy's choice for the next level.  One wave.
+// selector features and the policy's choice for the next level.  One block.
 // Features (all roughly in [0,1]), with F = frontier vertices, mf = their
 // out-edges, m = stored entries:
 //   0 log2(1+F)/log2(1+n)   1 F/n   2 log2(1+mf)/log2(1+m)   3 mf/m
 //   4 unvisited vertices/n  5 unvisited edges/m  6 nonzero frontier words/words
 //   7 min(level,64)/64      8 log2(1+mf/F)/16    9 log2(1+m/n)/8
 //   10 log2(n)/32           11 previous level pushed (1) or pulled (0)
-__global__ __launch_bounds__(64) void k_select(albd_graph g, albd_state st, albd_select_args sel)
+constexpr int kSelectBlock = 256;
+
+__global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel)
 {
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
-    const int lane = lane_id();
+    __shared__ unsigned long long s_red[6][kSelectBlock / 64];
+    const int lane = lane_id(), wid = threadIdx.x >> 6, tid = threadIdx.x;
     unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
-    for (int i = lane; i < ALBD_NSLOT; i += 64) {
+    for (int i = tid; i < ALBD_NSLOT; i += kSelectBlock) {
         const albd_slot &sl = st.slots[i];
         F += sl.next_count;
         mf += sl.next_edges;
@@ -335,10 +361,28 @@ __global__ __launch_bounds__(64) void k_select(albd_graph g, albd_state st, albd
     scn = wave_sum(scn);
     wds = wave_sum(wds);
     byp = wave_sum(byp);
+    if (lane == 0) {
+        s_red[0][wid] = F;
+        s_red[1][wid] = mf;
+        s_red[2][wid] = nzw;
+        s_red[3][wid] = scn;
+        s_red[4][wid] = wds;
+        s_red[5][wid] = byp;
+    }
+    __syncthreads();
+    F = mf = nzw = scn = wds = byp = 0;
+    for (int i = 0; i < kSelectBlock / 64; ++i) {
+        F += s_red[0][i];
+        mf += s_red[1][i];
+        nzw += s_red[2][i];
+        scn += s_red[3][i];
+        wds += s_red[4][i];
+        byp += s_red[5][i];
+    }
     const unsigned long long reached = st.ctr->reached + F;
     const unsigned long long redges = st.ctr->reached_edges + mf;
     const double n = (double)g.n, m = g.nnz > 0 ? (double)g.nnz : 1.0;
-    if (lane == 0) {
+    if (tid == 0) {
         xs[0] = (float)(log2(1.0 + (double)F) / log2(1.0 + n));
         xs[1] = (float)((double)F / n);
         xs[2] = (float)(log2(1.0 + (double)mf) / log2(1.0 + m));
@@ -358,25 +402,25 @@ __global__ __launch_bounds__(64) void k_select(albd_graph g, albd_state st, albd
         // W1[32][12] b1[32] W2[32][32] b2[32] W3[4][32] b3[4]: lane j owns unit j
         const float *W1 = sel.params, *B1 = W1 + 32 * 12, *W2 = B1 + 32, *B2 = W2 + 32 * 32,
                     *W3 = B2 + 32, *B3 = W3 + 4 * 32;
-        if (lane < ALBBA_MLP_H1) {
-            float a = B1[lane];
-            for (int i = 0; i < ALBBA_MLP_IN; ++i) a = fmaf(W1[lane * ALBBA_MLP_IN + i], xs[i], a);
-            h1[lane] = a > 0.f ? a : 0.f;
+        if (tid < ALBBA_MLP_H1) {
+            float a = B1[tid];
+            for (int i = 0; i < ALBBA_MLP_IN; ++i) a = fmaf(W1[tid * ALBBA_MLP_IN + i], xs[i], a);
+            h1[tid] = a > 0.f ? a : 0.f;
         }
         __syncthreads();
-        if (lane < ALBBA_MLP_H2) {
-            float a = B2[lane];
-            for (int i = 0; i < ALBBA_MLP_H1; ++i) a = fmaf(W2[lane * ALBBA_MLP_H1 + i], h1[i], a);
-            h2[lane] = a > 0.f ? a : 0.f;
+        if (tid < ALBBA_MLP_H2) {
+            float a = B2[tid];
+            for (int i = 0; i < ALBBA_MLP_H1; ++i) a = fmaf(W2[tid * ALBBA_MLP_H1 + i], h1[i], a);
+            h2[tid] = a > 0.f ? a : 0.f;
         }
         __syncthreads();
-        if (lane < ALBBA_MLP_OUT) {
-            float a = B3[lane];
-            for (int i = 0; i < ALBBA_MLP_H2; ++i) a = fmaf(W3[lane * ALBBA_MLP_H2 + i], h2[i], a);
-            ys[lane] = a;
+        if (tid < ALBBA_MLP_OUT) {
+            float a = B3[tid];
+            for (int i = 0; i < ALBBA_MLP_H2; ++i) a = fmaf(W3[tid * ALBBA_MLP_H2 + i], h2[i], a);
+            ys[tid] = a;
         }
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
             float bv = 0.f;
             for (int k = 0; k < ALBBA_MLP_OUT; ++k) {
                 if (!((sel.valid_mask >> k) & 1) || !(ys[k] == ys[k])) continue; // NaN never wins
@@ -384,7 +428,7 @@ __global__ __launch_bounds__(64) void k_select(albd_graph g, albd_state st, albd
             }
         }
     }
-    if (lane == 0) {
+    if (tid == 0) {
         // bfs/src/bfs_SELLCSv2.c:282-284: pull when F > (int)(n * 0.04)
         const long long thr = (long long)(int)(n * 0.04);
         const int heur = (long long)F > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
@@ -429,25 +473,34 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
 template <bool FROM_LIST>
 __global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
 {
-    const int lane = lane_id();
+    __shared__ unsigned int s_cnt[kWavesPerBlock];
+    __shared__ unsigned long long s_base;
+    constexpr int kGroups = 4; // 64-item groups per wave per step
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t n_in = FROM_LIST ? (int64_t)st.ctr->active_len : (int64_t)g.nwords;
-    const int64_t ngroups = (n_in + 63) >> 6;
-    const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
-    for (int64_t grp = gw; grp < ngroups; grp += nw) {
-        const int64_t i = grp * 64 + lane;
-        int32_t w = -1;
-        bool keep = false;
-        if (i < n_in) {
-            w = FROM_LIST ? st.active[i] : (int32_t)i;
-            keep = (g.haslen[w] & ~st.visited[w]) != 0;
+    const int64_t step = (int64_t)kBlock * kGroups;
+    for (int64_t base = (int64_t)blockIdx.x * step; base < n_in; base += (int64_t)gridDim.x * step) {
+        int32_t w[kGroups];
+        uint64_t m[kGroups];
+        unsigned int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kGroups; ++k) {
+            const int64_t i = base + (int64_t)(wid * kGroups + k) * 64 + lane;
+            bool keep = false;
+            w[k] = -1;
+            if (i < n_in) {
+                w[k] = FROM_LIST ? st.active[i] : (int32_t)i;
+                keep = (g.haslen[w[k]] & ~st.visited[w[k]]) != 0;
+            }
+            m[k] = __ballot(keep);
+            cnt += (unsigned int)__popcll(m[k]);
         }
-        const uint64_t m = __ballot(keep);
-        if (m == 0) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&st.ctr->active_next, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0, 64);
-        if (keep) st.active_next[base + __popcll(m & lanemask_lt(lane))] = w;
+        unsigned long long off = block_reserve(&st.ctr->active_next, cnt, s_cnt, &s_base);
+#pragma unroll
+        for (int k = 0; k < kGroups; ++k) {
+            if ((m[k] >> lane) & 1ull) st.active_next[off + __popcll(m[k] & lanemask_lt(lane))] = w[k];
+            off += (unsigned long long)__popcll(m[k]);
+        }
     }
 }
 
@@ -472,24 +525,26 @@ template <bool FINALIZE>
 __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_state st,
                                                            int32_t level)
 {
-    const int lane = lane_id();
+    __shared__ unsigned int s_cnt[kWavesPerBlock];
+    __shared__ unsigned long long s_base;
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
     const uint64_t *words = FINALIZE ? st.next : st.front;
     int32_t *q = FINALIZE ? st.queue_next : st.queue;
     unsigned long long *qlen = FINALIZE ? &st.ctr->qnext_len : &st.ctr->queue_len;
     const int64_t ngroups = ((int64_t)g.nwords + 63) >> 6;
-    const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     unsigned long long cnt = 0, nzw = 0, edges = 0;
-    for (int64_t grp = gw; grp < ngroups; grp += nw) {
+    // block-uniform steps of one 64-word group per wave
+    for (int64_t gb = (int64_t)blockIdx.x * kWavesPerBlock; gb < ngroups;
+         gb += (int64_t)gridDim.x * kWavesPerBlock) {
+        const int64_t grp = gb + wid;
         const int64_t w = grp * 64 + lane;
-        const uint64_t word = (w < g.nwords) ? words[w] : 0ull;
+        const uint64_t word = (grp < ngroups && w < g.nwords) ? words[w] : 0ull;
         if (FINALIZE && word) {
             st.visited[w] |= word; // only this lane touches word w here
             cnt += __popcll(word);
             nzw += 1;
         }
         const uint64_t nzmask = __ballot(word != 0);
-        if (nzmask == 0) continue;
         uint64_t keepw = 0; // lane k holds the keep mask of word grp*64+k
         uint64_t mm = nzmask;
         while (mm) {
@@ -504,6 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
                 if (FINALIZE) {
                     st.level_pos[pp] = level + 1;
                     st.parent_pos[pp] = (int32_t)st.pmin[pp];
+                    st.pmin[pp] = 0xFFFFFFFFu;
                     edges += (unsigned long long)d;
                 }
             }
@@ -513,10 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
         unsigned long long total;
         const unsigned long long excl =
             wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(keepw), &total);
-        if (total == 0) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(qlen, total);
-        base = __shfl(base, 0, 64);
+        const unsigned long long base = block_reserve(qlen, (unsigned int)total, s_cnt, &s_base);
         uint64_t km = __ballot(keepw != 0);
         while (km) {
             const int k = __ffsll((unsigned long long)km) - 1;
@@ -542,8 +595,9 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_s
     for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < g.n;
          v += (int64_t)gridDim.x * kBlock) {
         const int32_t p = g.pos_of[v];
-        st.level[v] = st.level_pos[p];
-        st.parent[v] = st.parent_pos[p];
+        const bool reached = (st.visited[p >> 6] >> (p & 63)) & 1ull;
+        st.level[v] = reached ? st.level_pos[p] : -1;
+        st.parent[v] = reached ? st.parent_pos[p] : -1;
     }
 }
 
@@ -1228,7 +1282,7 @@ int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int3
 
 int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st, const albd_select_args *sel)
 {
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, albd_stream(c), *g, *st, *sel);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel);
     HIPCHK(hipGetLastError());
     return 0;
 }

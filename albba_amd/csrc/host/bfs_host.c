@@ -38,6 +38,7 @@ struct albba_graph {
     void *scratch_visited; /* BEST-policy snapshots */
     void *scratch_ctr;
     int64_t *host_log;     /* small-frontier level sizes, 4 per level */
+    int needs_reset;       /* a BFS failed midway: pmin/pkey may be dirty */
 };
 
 /* ---- context ---- */
@@ -276,6 +277,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         (rc = albd_h2d(ctx->d, (void *)g->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n)) ||
         (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
         (rc = albd_fill_u64(ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
+        (rc = albd_fill_u32(ctx->d, st->pmin, 0xFFFFFFFFu, (size_t)npos)) ||
         (rc = albd_sync(ctx->d))) {
         char buf[512];
         snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "host staging buffer" : albd_error());
@@ -460,8 +462,28 @@ static int cmp_float(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
+static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
+                   albba_bfs_stats *stats);
+
+/* pmin and the small-frontier keys are all ones at rest (the kernels restore
+ * what they use); after a failed BFS they are refilled before the next one */
 int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
               albba_bfs_stats *stats)
+{
+    if (ctx && G && G->ctx == ctx && G->needs_reset) {
+        int rc = albd_fill_u32(ctx->d, G->st.pmin, 0xFFFFFFFFu, (size_t)G->g.npos);
+        if (!rc) rc = albd_fill_u64(ctx->d, G->st.pkey, ~0ull, (size_t)G->g.npos);
+        if (!rc) rc = albd_sync(ctx->d);
+        if (rc) return albba_fail(rc, "bfs: %s", albd_error());
+        G->needs_reset = 0;
+    }
+    int rc = bfs_run(ctx, G, root, opts_in, stats);
+    if (rc && rc != ALBBA_EINVAL && G) G->needs_reset = 1;
+    return rc;
+}
+
+static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
+                   albba_bfs_stats *stats)
 {
     if (!ctx || !G || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "bfs: bad context/graph");
     if (root < 0 || root >= G->n) return albba_fail(ALBBA_EINVAL, "bfs: root %d out of range [0,%d)", root, G->n);
