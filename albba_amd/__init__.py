@@ -96,6 +96,13 @@ class _Stats(C.Structure):
                 ("entries", I64P), ("words", I64P), ("bypassed", I64P), ("kernel_ms", F32P)]
 
 
+class _BatchResult(C.Structure):
+    _fields_ = [("num_levels", C.c_int32), ("error", C.c_int32), ("reached", C.c_int64),
+                ("edges_in_component", C.c_int64), ("ms", C.c_double), ("done_ms", C.c_double),
+                ("push_levels", C.c_int32), ("pull_levels", C.c_int32), ("team_size", C.c_int32),
+                ("team", C.c_int32)]
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -137,6 +144,12 @@ def lib() -> C.CDLL:
                                 C.POINTER(_Stats)]),
         "albba_bfs_results": (C.c_int, [C.c_void_p, C.c_void_p, I32P, I32P]),
         "albba_selector_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+        "albba_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
+                                         C.POINTER(C.c_void_p)]),
+        "albba_batch_free": (None, [C.c_void_p]),
+        "albba_batch_run": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
+                                      C.POINTER(_BatchResult), C.POINTER(C.c_float)]),
+        "albba_batch_results": (C.c_int, [C.c_void_p, C.c_int32, I32P, I32P]),
         "albba_selector_free": (None, [C.c_void_p]),
         "albba_selector_get_params": (C.c_int, [C.c_void_p, F32P]),
         "albba_selector_set_params": (C.c_int, [C.c_void_p, F32P]),
@@ -448,6 +461,63 @@ class DeviceGraph:
         pa = np.empty(self.n, np.int32)
         _check(lib().albba_bfs_results(self.ctx._p, self._p, lv.ctypes.data_as(I32P),
                                        pa.ctypes.data_as(I32P)))
+        return lv, pa
+
+
+MAP_LANE_ROW, MAP_WAVE_ROW = 0, 1
+
+
+class Batch:
+    """Batched BFS: one instance per entry of `graphs` (device graphs; one
+    may appear several times), all run in one cooperative launch."""
+
+    def __init__(self, ctx: Context, graphs):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        arr = (C.c_void_p * len(self.graphs))(*[dg._p.value for dg in self.graphs])
+        self._p = C.c_void_p()
+        _check(lib().albba_batch_create(ctx._p, arr, len(self.graphs), C.byref(self._p)))
+        ctx._adopt(self)
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value and _lib is not None:
+            _lib.albba_batch_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def run(self, roots, policy: int = SWITCH_HEURISTIC, selector: "Selector" = None,
+            mapping=None, workgroups: int = 0) -> dict:
+        """BFS from roots[i] on instance i; per-instance results as arrays."""
+        n = len(self.graphs)
+        r = np.ascontiguousarray(roots, dtype=np.int32)
+        if r.size != n:
+            raise ValueError("one root per instance")
+        o = _Opts()
+        lib().albba_bfs_opts_default(C.byref(o))
+        o.switch_policy = policy
+        if selector is not None:
+            o.selector = selector._p
+        mp = None
+        if mapping is not None:
+            mp = np.ascontiguousarray(np.broadcast_to(mapping, (n,)), dtype=np.int32)
+        res = (_BatchResult * n)()
+        ms = C.c_float()
+        _check(lib().albba_batch_run(self._p, r.ctypes.data_as(I32P), C.byref(o),
+                                     mp.ctypes.data_as(I32P) if mp is not None else None,
+                                     int(workgroups), res, C.byref(ms)))
+        out = {"total_ms": float(ms.value)}
+        for name, _ in _BatchResult._fields_:
+            out[name] = np.array([getattr(x, name) for x in res])
+        return out
+
+    def results(self, i: int):
+        g = self.graphs[i]
+        lv = np.empty(g.n, np.int32)
+        pa = np.empty(g.n, np.int32)
+        _check(lib().albba_batch_results(self._p, int(i), lv.ctypes.data_as(I32P),
+                                         pa.ctypes.data_as(I32P)))
         return lv, pa
 
 

@@ -207,6 +207,47 @@ ALBD_HIDDEN int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_sta
 ALBD_HIDDEN int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st,
                             const albd_select_args *sel);
 
+/* ---- batched BFS (batch.hip) ---- */
+/* One instance: a device graph, its own scratch, a root.  The control block
+ * ctl (32 words) must be zero before the launch: [0] team barrier,
+ * [1] found-list length, [2] error flag, [4..19] ring of four level counters
+ * (found, out-edges, queue entries, nonzero words), [24..31] results. */
+typedef struct albd_batch_item {
+    albd_graph g;
+    uint64_t *visited;           /* [nwords] */
+    uint64_t *bm[3];             /* [nwords] frontier bitmaps, rotating */
+    int32_t *qa, *qb;            /* [npos] frontier queues */
+    int32_t *found;              /* [npos] vertices found by push levels, in order */
+    unsigned long long *key;     /* [npos] (level, parent id), all ones at rest */
+    int32_t *level, *parent;     /* [n] results in vertex order */
+    unsigned long long *ctl;     /* [32] see above */
+    int32_t root;                /* vertex id */
+    int32_t mapping;             /* pull: ALBD_MAP_LANE_ROW or ALBD_MAP_WAVE_ROW */
+} albd_batch_item;
+
+enum { ALBD_MAP_LANE_ROW = 0, ALBD_MAP_WAVE_ROW = 1 };
+/* results in ctl[24..31] */
+enum { ALBD_RES_LEVELS = 24, ALBD_RES_REACHED, ALBD_RES_EDGES, ALBD_RES_T0, ALBD_RES_T1,
+       ALBD_RES_PUSHES, ALBD_RES_PULLS, ALBD_RES_ERROR };
+
+typedef struct albd_batch_args {
+    const albd_batch_item *items; /* [count] device */
+    const int32_t *wg_team;      /* [workgroups] team of each workgroup */
+    const int32_t *team_first;   /* [teams] first workgroup of the team */
+    const int32_t *team_size;    /* [teams] workgroups in the team */
+    const int32_t *team_ptr;     /* [teams+1] range of team_items per team */
+    const int32_t *team_items;   /* [count] instances, run in this order per team */
+    const float *params;         /* MLP parameters (learned policy) or NULL (heuristic) */
+    uint32_t spin_limit;         /* polls before a team barrier gives up */
+} albd_batch_args;
+
+/* workgroups that can be resident at once for albd_batch_launch */
+ALBD_HIDDEN int albd_batch_capacity(albd_ctx *c, int *workgroups);
+/* one cooperative launch of `workgroups` workgroups over all teams */
+ALBD_HIDDEN int albd_batch_launch(albd_ctx *c, const albd_batch_args *a, int32_t workgroups);
+/* ticks per millisecond of the device clock read by the batch kernel */
+ALBD_HIDDEN double albd_clock_ticks_per_ms(albd_ctx *c);
+
 /* ---- selector MLP (mlp.hip); every pointer is device memory ---- */
 #define ALBD_MLP_SCRATCH_PER_SAMPLE (32 + 32 + 4 + 32 + 32 + 1)
 /* out[ns*4] = MLP(x[ns*12]) */

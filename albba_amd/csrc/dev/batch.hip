@@ -1,0 +1,477 @@
+// Batched BFS: many instances (graphs and roots of uneven size) in one
+// cooperative launch.  The workgroups are split into teams; a team runs the
+// complete level loop of its instances one after the other, with one team
+// barrier per level and the direction of every level chosen on the device
+// (the reference heuristic, bfs/src/bfs_SELLCSv2.c:282-284, or the learned
+// selector's MLP), so no instance ever waits for the host.
+//
+// Exactness, as in the small-frontier kernel: every position has a 64-bit
+// key (level << 32 | parent id), all ones at rest.  A push edge u -> v at
+// level L does atomicMin(key[v], L << 32 | id(u)); a pull lane that finds
+// its row's first frontier neighbour u does the same.  The key keeps the
+// smallest (level, parent) pair, so it holds the BFS level and the min-id
+// parent no matter in which order the team's waves run, and the old value
+// names the one discoverer.  The visited bitmap is only a filter; the
+// frontier bitmaps are exact (a pull relies on them for the min-id parent).
+//
+// Frontier bitmaps B_L (vertices of level L) rotate over three buffers: at
+// level L the team reads B_L, sets B_{L+1} and clears B_{L-1}, which becomes
+// B_{L+2}.  A push level also lists its discoveries (found list) so that
+// their visited bits and, two levels later, their bitmap bits are handled
+// without a full pass; a pull level updates visited at once and its bitmap
+// is cleared by a full pass.
+#include "dev_common.h"
+#include "select_dev.h"
+#include "../../../include/albba.h"
+
+namespace {
+
+constexpr int kBB = 512;              // threads per workgroup
+constexpr int kBWaves = kBB / 64;
+constexpr int kBChunk = kBB;          // queue entries staged per pass
+constexpr int kBPer = 4;              // edges per thread per window
+constexpr int kBWin = kBPer * kBB;    // edges per window
+
+struct BLds {
+    int64_t pre[kBChunk];
+    int64_t start[kBChunk];
+    uint32_t ou[kBChunk];
+    int64_t scan[kBWaves];
+    int32_t found[kBWin];
+    int32_t queue[kBWin];
+    unsigned int nf, nq;
+    unsigned long long bf, bq;
+    unsigned long long red[4][kBWaves];
+    unsigned int s_cnt[kBWaves];
+    unsigned long long s_base;
+    int ok, choice;
+    float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
+};
+
+__device__ __forceinline__ unsigned int *err_flag(const albd_batch_item &it)
+{
+    return reinterpret_cast<unsigned int *>(&it.ctl[2]);
+}
+
+__device__ __forceinline__ long long *ring(const albd_batch_item &it, int L)
+{
+    return reinterpret_cast<long long *>(it.ctl) + 4 + 4 * (L & 3);
+}
+
+// per-wave counts -> one atomic per workgroup into the level's ring slot
+__device__ void add_counts(BLds &S, long long *slot, unsigned long long f, unsigned long long e,
+                           unsigned long long q, unsigned long long w)
+{
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    f = wave_sum(f);
+    e = wave_sum(e);
+    q = wave_sum(q);
+    w = wave_sum(w);
+    if (lane == 0) {
+        S.red[0][wid] = f;
+        S.red[1][wid] = e;
+        S.red[2][wid] = q;
+        S.red[3][wid] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBWaves; ++i) t += S.red[threadIdx.x][i];
+        if (t) atomicAdd(reinterpret_cast<unsigned long long *>(&slot[threadIdx.x]), t);
+    }
+    __syncthreads();
+}
+
+// LDS found/queue lists of a push window -> the instance's global lists
+__device__ void flush_lists(BLds &S, const albd_batch_item &it, long long *slot, int32_t *qb)
+{
+    __syncthreads();
+    const unsigned int nf = S.nf, nq = S.nq;
+    if (threadIdx.x == 0) {
+        S.bf = nf ? atomicAdd(&it.ctl[1], (unsigned long long)nf) : 0;
+        S.bq = nq ? atomicAdd(reinterpret_cast<unsigned long long *>(&slot[2]), (unsigned long long)nq) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        S.nf = 0;
+        S.nq = 0;
+    }
+    const unsigned long long bf = S.bf, bq = S.bq;
+    for (unsigned int i = threadIdx.x; i < nf; i += kBB) st_coherent(&it.found[bf + i], S.found[i]);
+    for (unsigned int i = threadIdx.x; i < nq; i += kBB) st_coherent(&qb[bq + i], S.queue[i]);
+    __syncthreads();
+}
+
+// One BFS of one instance by a team of T workgroups (this one is `rank`).
+// Returns false when a team barrier gave up.
+__device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T,
+                             const float *params, uint32_t spin_limit)
+{
+    const albd_graph &g = it.g;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    const int64_t gtid = (int64_t)rank * kBB + tid, gthreads = (int64_t)T * kBB;
+    const int64_t nwords = g.nwords;
+    unsigned long long *bar = &it.ctl[0];
+    unsigned int *err = err_flag(it);
+    const bool leader = rank == 0 && tid == 0;
+    if (leader) st_coherent(reinterpret_cast<long long *>(&it.ctl[ALBD_RES_T0]), (long long)wall_clock64());
+    for (int64_t i = gtid; i < nwords; i += gthreads) {
+        st_coherent(&it.visited[i], (uint64_t)0);
+        st_coherent(&it.bm[0][i], (uint64_t)0);
+        st_coherent(&it.bm[1][i], (uint64_t)0);
+        st_coherent(&it.bm[2][i], (uint64_t)0);
+    }
+    if (tid == 0) {
+        S.nf = 0;
+        S.nq = 0;
+    }
+    if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
+    const int32_t rp = g.pos_of[it.root];
+    const int64_t rdeg = g.push_ptr[rp + 1] - g.push_ptr[rp];
+    if (leader) {
+        // the root: level 0, its own parent, frontier B_0, found entry 0
+        const uint64_t bit = 1ull << (rp & 63);
+        st_coherent(&it.key[rp], (unsigned long long)(uint32_t)it.root);
+        atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[rp >> 6]), bit);
+        atomicOr(reinterpret_cast<unsigned long long *>(&it.bm[0][rp >> 6]), bit);
+        st_coherent(&it.found[0], rp);
+        st_coherent(&it.ctl[1], 1ull);
+        if (rdeg > 0) st_coherent(&it.qa[0], rp);
+    }
+    if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
+
+    // team-uniform loop state
+    int32_t *qa = it.qa, *qb = it.qb;
+    int64_t qlen = rdeg > 0 ? 1 : 0;
+    bool have_queue = true;                 // qa holds B_L's vertices of out-degree > 0
+    unsigned long long qbase = 0;           // ctl[3] at the last barrier
+    unsigned long long F = 1, mf = (unsigned long long)rdeg, nzw = 1;
+    unsigned long long reached = 1, redges = (unsigned long long)rdeg;
+    int prev_choice = ALBBA_CHOICE_PUSH;
+    // found-list range of B_L (pushed vertices) and of B_{L-1}; how each bitmap was made
+    int64_t fs_cur = 0, fe_cur = 1, fs_prev = 0, fe_prev = 0;
+    bool cur_by_pull = false, prev_by_pull = false;
+    long long pushes = 0, pulls = 0;
+    int L = 0;
+    for (; F > 0; ++L) {
+        uint64_t *bcur = it.bm[L % 3], *bnext = it.bm[(L + 1) % 3], *bprev = it.bm[(L + 2) % 3];
+        long long *slot = ring(it, L);
+        // direction of this level (the same in every workgroup of the team)
+        int choice = heuristic_choice(F, g.n);
+        if (params) {
+            if (tid == 0)
+                level_features(S.xs, g.n, g.nnz, nwords, F, mf, nzw, reached, redges, L, prev_choice);
+            __syncthreads();
+            const int c = mlp_argmin(params, S.xs, S.h1, S.h2, S.ys, 0xF, &S.choice);
+            if (c >= 0) choice = c;
+        }
+        const bool push = choice == ALBBA_CHOICE_PUSH;
+        // the slot two levels ahead is free: its last readers passed a barrier
+        if (leader)
+            for (int f = 0; f < 4; ++f) st_coherent(ring(it, L + 2) + f, 0ll);
+        // B_{L-1} goes back to zero (it becomes B_{L+2})
+        if (L >= 1) {
+            if (prev_by_pull) {
+                for (int64_t i = gtid; i < nwords; i += gthreads) st_coherent(&bprev[i], (uint64_t)0);
+            } else {
+                for (int64_t i = fs_prev + gtid; i < fe_prev; i += gthreads)
+                    st_coherent(&bprev[ld_coherent(&it.found[i]) >> 6], (uint64_t)0);
+            }
+        }
+        // pushed vertices of B_L join visited (a pulled level did it at once)
+        if (!cur_by_pull)
+            for (int64_t i = fs_cur + gtid; i < fe_cur; i += gthreads) {
+                const int32_t v = ld_coherent(&it.found[i]);
+                atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[v >> 6]), 1ull << (v & 63));
+            }
+        const unsigned long long lkey = (unsigned long long)(uint32_t)(L + 1) << 32;
+        unsigned long long my_f = 0, my_e = 0, my_w = 0;
+        if (push) {
+            ++pushes;
+            if (!have_queue) {
+                // queue of B_L from its bitmap (the last level pulled); slots
+                // are reserved once per block step from ctl[3], a counter that
+                // only grows (qbase: its value at the last barrier)
+                for (int64_t wb = (int64_t)rank * kBB; wb < nwords; wb += gthreads) {
+                    const int64_t w = wb + tid;
+                    const uint64_t word = w < nwords ? ld_coherent(&bcur[w]) : 0ull;
+                    uint64_t keep = 0, bits = word;
+                    while (bits) {
+                        const int b = __ffsll((unsigned long long)bits) - 1;
+                        bits &= bits - 1;
+                        const int64_t p = w * 64 + b;
+                        if (g.push_ptr[p + 1] > g.push_ptr[p]) keep |= 1ull << b;
+                    }
+                    unsigned long long wtot;
+                    const unsigned long long ex =
+                        wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(keep), &wtot);
+                    if (lane == 0) S.s_cnt[wid] = (unsigned int)wtot;
+                    __syncthreads();
+                    if (tid == 0) {
+                        unsigned long long tot = 0;
+                        for (int i = 0; i < kBWaves; ++i) tot += S.s_cnt[i];
+                        S.s_base = tot ? atomicAdd(&it.ctl[3], tot) : 0ull;
+                    }
+                    __syncthreads();
+                    unsigned long long o = S.s_base - qbase + ex;
+                    for (int i = 0; i < wid; ++i) o += S.s_cnt[i];
+                    __syncthreads();
+                    uint64_t kb = keep;
+                    while (kb) {
+                        const int b = __ffsll((unsigned long long)kb) - 1;
+                        kb &= kb - 1;
+                        st_coherent(&qa[o++], (int32_t)(w * 64 + b));
+                    }
+                }
+                if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
+                const unsigned long long qend = ld_coherent(&it.ctl[3]);
+                qlen = (int64_t)(qend - qbase);
+                qbase = qend;
+            }
+            // expansion: workgroup `rank` takes queue entries [q0, q1)
+            const int64_t per = (qlen + T - 1) / T;
+            const int64_t q0 = (int64_t)rank * per, q1 = q0 + per < qlen ? q0 + per : qlen;
+            for (int64_t base = q0; base < q1; base += kBChunk) { // uniform in the workgroup
+                const int cnt_e = (int)(q1 - base < kBChunk ? q1 - base : kBChunk);
+                int64_t a0 = 0, e0 = 0;
+                uint32_t o0 = 0;
+                if (tid < cnt_e) {
+                    const int32_t u = ld_coherent(&qa[base + tid]);
+                    a0 = g.push_ptr[u];
+                    e0 = g.push_ptr[u + 1];
+                    o0 = (uint32_t)g.row_order[u];
+                }
+                int64_t Te;
+                const int64_t ex = block_scan_excl<kBB>(e0 - a0, &Te, S.scan);
+                S.pre[tid] = ex;
+                S.start[tid] = a0;
+                S.ou[tid] = o0;
+                __syncthreads();
+                for (int64_t wb = 0; wb < Te; wb += kBWin) { // uniform
+                    int64_t e = wb + (int64_t)tid * kBPer;
+                    const int64_t e_end = e + kBPer < Te ? e + kBPer : Te;
+                    int lo = 0;
+                    if (e < e_end) {
+                        int hi = cnt_e;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (S.pre[mid] <= e) lo = mid;
+                            else hi = mid;
+                        }
+                    }
+                    int32_t v[kBPer];
+                    uint32_t ou[kBPer];
+                    bool okv[kBPer];
+#pragma unroll
+                    for (int i = 0; i < kBPer; ++i) {
+                        okv[i] = e < e_end;
+                        v[i] = 0;
+                        ou[i] = 0;
+                        if (okv[i]) {
+                            while (lo + 1 < cnt_e && S.pre[lo + 1] <= e) ++lo;
+                            v[i] = g.push_col[S.start[lo] + (e - S.pre[lo])];
+                            ou[i] = S.ou[lo];
+                            ++e;
+                        }
+                    }
+                    uint64_t vis[kBPer];
+#pragma unroll
+                    for (int i = 0; i < kBPer; ++i) vis[i] = okv[i] ? it.visited[v[i] >> 6] : ~0ull;
+                    bool won[kBPer];
+#pragma unroll
+                    for (int i = 0; i < kBPer; ++i) {
+                        won[i] = false;
+                        if (!((vis[i] >> (v[i] & 63)) & 1ull)) {
+                            const unsigned long long old = atomicMin(&it.key[v[i]], lkey | ou[i]);
+                            won[i] = old == ~0ull;
+                        }
+                    }
+                    int64_t dv[kBPer];
+#pragma unroll
+                    for (int i = 0; i < kBPer; ++i) {
+                        dv[i] = 0;
+                        if (won[i]) {
+                            dv[i] = g.push_ptr[v[i] + 1] - g.push_ptr[v[i]];
+                            const unsigned long long old = atomicOr(
+                                reinterpret_cast<unsigned long long *>(&bnext[v[i] >> 6]), 1ull << (v[i] & 63));
+                            my_w += old == 0 ? 1 : 0;
+                            my_f += 1;
+                            my_e += (unsigned long long)dv[i];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < kBPer; ++i) {
+                        const unsigned int pf = wave_append(&S.nf, won[i]);
+                        if (won[i]) S.found[pf] = v[i];
+                        const unsigned int pq = wave_append(&S.nq, dv[i] > 0);
+                        if (dv[i] > 0) S.queue[pq] = v[i];
+                    }
+                    flush_lists(S, it, slot, qb);
+                }
+                __syncthreads(); // pre/start/ou are restaged
+            }
+        } else {
+            ++pulls;
+            // pull: workgroup `rank` takes words [w0, w1); wave wid takes every
+            // kBWaves-th word; unvisited lanes look for their row's first
+            // neighbour in B_L (rows list neighbours by ascending id)
+            const int64_t per = (nwords + T - 1) / T;
+            const int64_t w0 = (int64_t)rank * per, w1 = w0 + per < nwords ? w0 + per : nwords;
+            for (int64_t w = w0 + wid; w < w1; w += kBWaves) {
+                const uint64_t hl = g.haslen[w];
+                if (hl == 0) continue;
+                const uint64_t seen = ld_coherent(&it.visited[w]) | ld_coherent(&bcur[w]);
+                const uint64_t unres = hl & ~seen;
+                if (unres == 0) continue;
+                const int64_t p = w * 64 + lane;
+                int32_t par = -1;
+                if (it.mapping == ALBD_MAP_LANE_ROW) {
+                    if ((unres >> lane) & 1ull) {
+                        const int64_t r1 = g.pull_ptr[p + 1];
+                        for (int64_t k = g.pull_ptr[p]; k < r1; ++k) {
+                            const int32_t u = g.pull_col[k];
+                            if ((ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull) {
+                                par = g.row_order[u];
+                                break;
+                            }
+                        }
+                    }
+                } else {
+                    // the wave scans one row at a time, 64 entries per step
+                    uint64_t todo = unres;
+                    while (todo) {
+                        const int r = __ffsll((unsigned long long)todo) - 1;
+                        todo &= todo - 1;
+                        const int64_t rs = g.pull_ptr[w * 64 + r], re = g.pull_ptr[w * 64 + r + 1];
+                        for (int64_t k = rs; k < re; k += 64) {
+                            const int64_t kk = k + lane;
+                            bool h = false;
+                            int32_t u = 0;
+                            if (kk < re) {
+                                u = g.pull_col[kk];
+                                h = (ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull;
+                            }
+                            const uint64_t hm = __ballot(h);
+                            if (hm) {
+                                const int32_t first = __shfl(u, __ffsll((unsigned long long)hm) - 1, 64);
+                                if (lane == r) par = g.row_order[first];
+                                break;
+                            }
+                        }
+                    }
+                }
+                bool won = false;
+                if (par >= 0) {
+                    const unsigned long long old = atomicMin(&it.key[p], lkey | (uint32_t)par);
+                    won = old == ~0ull;
+                }
+                const uint64_t fm = __ballot(won);
+                if (fm) {
+                    if (lane == 0) {
+                        const unsigned long long old =
+                            atomicOr(reinterpret_cast<unsigned long long *>(&bnext[w]), fm);
+                        atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[w]), fm);
+                        my_w += old == 0 ? 1 : 0;
+                    }
+                    if (won) {
+                        my_f += 1;
+                        my_e += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
+                    }
+                }
+            }
+        }
+        add_counts(S, slot, my_f, my_e, 0, my_w);
+        if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
+        // B_{L+1} is complete: its size, out-edges, words and (push) queue
+        F = (unsigned long long)ld_coherent(&slot[0]);
+        mf = (unsigned long long)ld_coherent(&slot[1]);
+        nzw = (unsigned long long)ld_coherent(&slot[3]);
+        reached += F;
+        redges += mf;
+        fs_prev = fs_cur;
+        fe_prev = fe_cur;
+        prev_by_pull = cur_by_pull;
+        if (push) {
+            fs_cur = fe_cur;
+            fe_cur = (int64_t)ld_coherent(&it.ctl[1]);
+            qlen = ld_coherent(&slot[2]);
+            int32_t *t = qa;
+            qa = qb;
+            qb = t;
+            have_queue = true;
+            cur_by_pull = false;
+        } else {
+            fs_cur = fe_cur; // no found entries for a pulled level
+            have_queue = false;
+            cur_by_pull = true;
+        }
+        prev_choice = push ? ALBBA_CHOICE_PUSH : ALBBA_CHOICE_PULL;
+    }
+    // results in vertex order from the keys; the keys go back to all ones
+    for (int64_t v = gtid; v < g.n; v += gthreads) {
+        const int32_t p = g.pos_of[v];
+        const unsigned long long k = ld_coherent(&it.key[p]);
+        const bool hit = k != ~0ull;
+        it.level[v] = hit ? (int32_t)(k >> 32) : -1;
+        it.parent[v] = hit ? (int32_t)(uint32_t)k : -1;
+        if (hit) st_coherent(&it.key[p], ~0ull);
+    }
+    if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
+    if (leader) {
+        long long *res = reinterpret_cast<long long *>(it.ctl);
+        st_coherent(&res[ALBD_RES_LEVELS], (long long)L);
+        st_coherent(&res[ALBD_RES_REACHED], (long long)reached);
+        st_coherent(&res[ALBD_RES_EDGES], (long long)redges);
+        st_coherent(&res[ALBD_RES_PUSHES], pushes);
+        st_coherent(&res[ALBD_RES_PULLS], pulls);
+        st_coherent(&res[ALBD_RES_T1], (long long)wall_clock64());
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kBB) void k_bfs_batch(albd_batch_args a)
+{
+    __shared__ BLds S;
+    const int t = a.wg_team[blockIdx.x];
+    const int rank = (int)blockIdx.x - a.team_first[t], T = a.team_size[t];
+    for (int j = a.team_ptr[t]; j < a.team_ptr[t + 1]; ++j) {
+        const albd_batch_item &it = a.items[a.team_items[j]];
+        if (!run_instance(S, it, rank, T, a.params, a.spin_limit)) {
+            if (threadIdx.x == 0)
+                st_coherent(reinterpret_cast<long long *>(it.ctl) + ALBD_RES_ERROR, 1ll);
+            break;
+        }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int albd_batch_capacity(albd_ctx *c, int *workgroups)
+{
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs_batch, kBB, 0));
+    *workgroups = per_cu * albd_num_cus(c);
+    return 0;
+}
+
+int albd_batch_launch(albd_ctx *c, const albd_batch_args *a, int32_t workgroups)
+{
+    albd_batch_args args = *a;
+    void *p[] = {&args};
+    HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_bfs_batch), dim3(workgroups),
+                                      dim3(kBB), p, 0, albd_stream(c)));
+    return 0;
+}
+
+double albd_clock_ticks_per_ms(albd_ctx *c)
+{
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, albd_ctx_device(c)) != hipSuccess ||
+        khz <= 0)
+        return 100000.0; // the 100 MHz constant clock of CDNA
+    return (double)khz;
+}
+
+} // extern "C"

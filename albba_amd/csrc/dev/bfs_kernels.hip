@@ -19,6 +19,7 @@
 // at the start of the level does atomicMin(pmin[v], id(u)) and sets v in the
 // next bitmap; a word pass then writes level/parent and builds the next queue.
 #include "dev_common.h"
+#include "select_dev.h"
 #include "../../../include/albba.h"
 
 #include <climits>
@@ -331,19 +332,15 @@ __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 }
 
 // End of a level: the frontier it produced, the running totals, the
-// selector features and the policy's choice for the next level.  One block.
-// Features (all roughly in [0,1]), with F = frontier vertices, mf = their
-// out-edges, m = stored entries:
-//   0 log2(1+F)/log2(1+n)   1 F/n   2 log2(1+mf)/log2(1+m)   3 mf/m
-//   4 unvisited vertices/n  5 unvisited edges/m  6 nonzero frontier words/words
-//   7 min(level,64)/64      8 log2(1+mf/F)/16    9 log2(1+m/n)/8
-//   10 log2(n)/32           11 previous level pushed (1) or pulled (0)
+// selector features (level_features) and the policy's choice for the next
+// level.  One block.
 constexpr int kSelectBlock = 256;
 
 __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel)
 {
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
     __shared__ unsigned long long s_red[6][kSelectBlock / 64];
+    __shared__ int s_choice;
     const int lane = lane_id(), wid = threadIdx.x >> 6, tid = threadIdx.x;
     unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
     for (int i = tid; i < ALBD_NSLOT; i += kSelectBlock) {
@@ -381,57 +378,13 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
     }
     const unsigned long long reached = st.ctr->reached + F;
     const unsigned long long redges = st.ctr->reached_edges + mf;
-    const double n = (double)g.n, m = g.nnz > 0 ? (double)g.nnz : 1.0;
-    if (tid == 0) {
-        xs[0] = (float)(log2(1.0 + (double)F) / log2(1.0 + n));
-        xs[1] = (float)((double)F / n);
-        xs[2] = (float)(log2(1.0 + (double)mf) / log2(1.0 + m));
-        xs[3] = (float)((double)mf / m);
-        xs[4] = (float)((n - (double)reached) / n);
-        xs[5] = (float)((m - (double)redges) / m);
-        xs[6] = (float)((double)nzw / (double)(g.nwords > 0 ? g.nwords : 1));
-        xs[7] = (float)((sel.level < 64 ? sel.level : 64) / 64.0);
-        xs[8] = (float)(log2(1.0 + (double)mf / (double)(F > 0 ? F : 1)) / 16.0);
-        xs[9] = (float)(log2(1.0 + m / n) / 8.0);
-        xs[10] = (float)(log2(n) / 32.0);
-        xs[11] = sel.prev_choice == ALBBA_CHOICE_PUSH ? 1.0f : 0.0f;
-    }
+    if (tid == 0)
+        level_features(xs, g.n, g.nnz, g.nwords, F, mf, nzw, reached, redges, sel.level,
+                       sel.prev_choice);
     __syncthreads();
-    int choice = -1;
-    if (sel.params) {
-        // W1[32][12] b1[32] W2[32][32] b2[32] W3[4][32] b3[4]: lane j owns unit j
-        const float *W1 = sel.params, *B1 = W1 + 32 * 12, *W2 = B1 + 32, *B2 = W2 + 32 * 32,
-                    *W3 = B2 + 32, *B3 = W3 + 4 * 32;
-        if (tid < ALBBA_MLP_H1) {
-            float a = B1[tid];
-            for (int i = 0; i < ALBBA_MLP_IN; ++i) a = fmaf(W1[tid * ALBBA_MLP_IN + i], xs[i], a);
-            h1[tid] = a > 0.f ? a : 0.f;
-        }
-        __syncthreads();
-        if (tid < ALBBA_MLP_H2) {
-            float a = B2[tid];
-            for (int i = 0; i < ALBBA_MLP_H1; ++i) a = fmaf(W2[tid * ALBBA_MLP_H1 + i], h1[i], a);
-            h2[tid] = a > 0.f ? a : 0.f;
-        }
-        __syncthreads();
-        if (tid < ALBBA_MLP_OUT) {
-            float a = B3[tid];
-            for (int i = 0; i < ALBBA_MLP_H2; ++i) a = fmaf(W3[tid * ALBBA_MLP_H2 + i], h2[i], a);
-            ys[tid] = a;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            float bv = 0.f;
-            for (int k = 0; k < ALBBA_MLP_OUT; ++k) {
-                if (!((sel.valid_mask >> k) & 1) || !(ys[k] == ys[k])) continue; // NaN never wins
-                if (choice < 0 || ys[k] < bv) { choice = k; bv = ys[k]; }
-            }
-        }
-    }
+    const int choice = sel.params ? mlp_argmin(sel.params, xs, h1, h2, ys, sel.valid_mask, &s_choice) : -1;
     if (tid == 0) {
-        // bfs/src/bfs_SELLCSv2.c:282-284: pull when F > (int)(n * 0.04)
-        const long long thr = (long long)(int)(n * 0.04);
-        const int heur = (long long)F > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
+        const int heur = heuristic_choice(F, g.n);
         st.ctr->frontier = F;
         st.ctr->frontier_edges = mf;
         st.ctr->frontier_words = nzw;
@@ -785,78 +738,9 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
 constexpr int kSmallBlock = 1024;
 constexpr int kSmallWaves = kSmallBlock / 64;
 
-template <class T> __device__ __forceinline__ T ld_coherent(const T *p)
-{
-    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// exclusive scan of one value per thread over the 1024-thread workgroup
-__device__ __forceinline__ int64_t small_block_scan(int64_t v, int64_t *total, int64_t *lds)
-{
-    const int lane = lane_id(), wid = threadIdx.x >> 6;
-    int64_t wt;
-    const int64_t ex = wave_exclusive_scan<int64_t>(v, &wt);
-    if (lane == 0) lds[wid] = wt;
-    __syncthreads();
-    int64_t off = 0, tot = 0;
-    for (int i = 0; i < kSmallWaves; ++i) {
-        if (i < wid) off += lds[i];
-        tot += lds[i];
-    }
-    *total = tot;
-    return ex + off;
-}
-
-// append one entry per predicated lane to an LDS list: returns the slot of
-// this lane (valid where pred holds); works under any exec mask
-__device__ __forceinline__ unsigned int wave_append(unsigned int *counter, bool pred)
-{
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return 0;
-    const int lane = lane_id();
-    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
-    base = __shfl(base, leader, 64);
-    return base + (unsigned int)__popcll(m & lanemask_lt(lane));
-}
-
 constexpr int kSmallChunk = kSmallBlock;      // queue entries staged per pass
 constexpr int kSmallPer = 4;                  // edges per thread per window
 constexpr int kSmallWin = kSmallPer * kSmallBlock;
-
-template <class T> __device__ __forceinline__ void st_coherent(T *p, T v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// grid-wide barrier over a counter that only grows (reset by the host before
-// the launch): wave 0 of each workgroup arrives and waits until all G have
-// arrived at this generation.  Gives up after spin_limit polls or when
-// another workgroup gave up, so every wave always reaches the kernel's end.
-__device__ __forceinline__ bool grid_sync(albd_persist *P, unsigned int nblocks,
-                                          unsigned int spin_limit, int *s_ok)
-{
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        const unsigned long long arrived = atomicAdd(&P->barrier, 1ull);
-        const unsigned long long target = (arrived / nblocks + 1) * nblocks;
-        unsigned int spins = 0;
-        while (ld_coherent(&P->barrier) < target) {
-            if (++spins > spin_limit || ld_coherent(&P->error) != 0) {
-                atomicOr(&P->error, 1u);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        *s_ok = ok;
-    }
-    __syncthreads();
-    return *s_ok != 0;
-}
 
 struct SmallArgs {
     albd_graph g;
@@ -923,7 +807,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                 o0 = (uint32_t)g.row_order[u];
             }
             int64_t T;
-            const int64_t ex = small_block_scan(e0 - a0, &T, s_scan);
+            const int64_t ex = block_scan_excl<kSmallBlock>(e0 - a0, &T, s_scan);
             s_pre[tid] = ex;
             s_start[tid] = a0;
             s_ou[tid] = o0;
@@ -1012,7 +896,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
             if (w == 0 && k + 1 < a.max_levels)
                 for (int f = 0; f < 4; ++f) st_coherent(&cnt[4 + f], (int64_t)0);
         }
-        ok = grid_sync(P, G, a.spin_limit, &s_ok);
+        ok = team_sync(&P->barrier, &P->error, G, a.spin_limit, &s_ok);
         if (!ok) break;
         const int64_t total = (int64_t)ld_coherent(&P->found_total);
         const int64_t na = total - fo;

@@ -1,0 +1,317 @@
+/*
+ * Batched BFS host side: per-instance device scratch, the split of the
+ * resident workgroups into teams, one cooperative launch, per-instance
+ * results from the device clock.
+ *
+ * Team split: with at most as many instances as workgroups every instance
+ * gets its own team, sized in proportion to its work (stored entries plus
+ * vertices), at least one workgroup; with more instances than workgroups,
+ * one-workgroup teams take the instances longest-first, each to the least
+ * loaded team.
+ */
+#include "internal.h"
+#include "device_int.h"
+#include "selector_int.h"
+
+#define CTL_WORDS 32
+
+struct albba_batch {
+    albba_ctx *ctx;
+    int32_t count;
+    albba_graph **graphs;
+    albd_batch_item *items;       /* host copies */
+    unsigned long long *ctl;      /* device: count * CTL_WORDS */
+    unsigned long long *hctl;     /* host mirror */
+    void **allocs;                /* device allocations to free */
+    int32_t nallocs;
+    int32_t *team_of;             /* instance -> team, team sizes of the last run */
+    int32_t *team_size_of;
+};
+
+static void *bdalloc(albba_batch *b, size_t bytes, int *rc)
+{
+    void *p = NULL;
+    if (*rc) return NULL;
+    *rc = albd_malloc(b->ctx->d, bytes ? bytes : 8, &p);
+    if (!*rc) b->allocs[b->nallocs++] = p;
+    return p;
+}
+
+void albba_batch_free(albba_batch *b)
+{
+    if (!b) return;
+    for (int32_t i = 0; i < b->nallocs; ++i) albd_free(b->ctx->d, b->allocs[i]);
+    free(b->allocs);
+    free(b->graphs);
+    free(b->items);
+    free(b->hctl);
+    free(b->team_of);
+    free(b->team_size_of);
+    free(b);
+}
+
+int albba_batch_create(albba_ctx *ctx, albba_graph *const *graphs, int32_t count, albba_batch **out)
+{
+    if (!ctx || !graphs || count <= 0 || !out) return albba_fail(ALBBA_EINVAL, "batch_create: bad argument");
+    for (int32_t i = 0; i < count; ++i)
+        if (!graphs[i] || graphs[i]->ctx != ctx)
+            return albba_fail(ALBBA_EINVAL, "batch_create: graph %d missing or on another context", i);
+    albba_batch *b = (albba_batch *)albba_calloc(1, sizeof(albba_batch));
+    if (!b) return albba_fail(ALBBA_ENOMEM, "batch_create");
+    b->ctx = ctx;
+    b->count = count;
+    b->graphs = (albba_graph **)albba_calloc((size_t)count, sizeof(albba_graph *));
+    b->items = (albd_batch_item *)albba_calloc((size_t)count, sizeof(albd_batch_item));
+    b->hctl = (unsigned long long *)albba_calloc((size_t)count * CTL_WORDS, sizeof(unsigned long long));
+    b->allocs = (void **)albba_calloc((size_t)count * 12 + 1, sizeof(void *));
+    b->team_of = (int32_t *)albba_calloc((size_t)count, sizeof(int32_t));
+    b->team_size_of = (int32_t *)albba_calloc((size_t)count, sizeof(int32_t));
+    if (!b->graphs || !b->items || !b->hctl || !b->allocs || !b->team_of || !b->team_size_of) {
+        albba_batch_free(b);
+        return albba_fail(ALBBA_ENOMEM, "batch_create");
+    }
+    int rc = 0;
+    b->ctl = (unsigned long long *)bdalloc(b, sizeof(unsigned long long) * CTL_WORDS * (size_t)count, &rc);
+    for (int32_t i = 0; i < count && !rc; ++i) {
+        albba_graph *G = graphs[i];
+        b->graphs[i] = G;
+        albd_batch_item *it = &b->items[i];
+        const size_t nw = (size_t)G->g.nwords, np = (size_t)G->g.npos, n = (size_t)G->g.n;
+        it->g = G->g;
+        it->visited = (uint64_t *)bdalloc(b, 8 * nw, &rc);
+        for (int k = 0; k < 3; ++k) it->bm[k] = (uint64_t *)bdalloc(b, 8 * nw, &rc);
+        it->qa = (int32_t *)bdalloc(b, 4 * np, &rc);
+        it->qb = (int32_t *)bdalloc(b, 4 * np, &rc);
+        it->found = (int32_t *)bdalloc(b, 4 * np, &rc);
+        it->key = (unsigned long long *)bdalloc(b, 8 * np, &rc);
+        it->level = (int32_t *)bdalloc(b, 4 * n, &rc);
+        it->parent = (int32_t *)bdalloc(b, 4 * n, &rc);
+        if (!rc) rc = albd_fill_u64(ctx->d, it->key, ~0ull, np);
+        it->ctl = b->ctl + (size_t)i * CTL_WORDS;
+        it->mapping = ALBD_MAP_LANE_ROW;
+    }
+    if (!rc) rc = albd_sync(ctx->d);
+    if (rc) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s", albd_error());
+        albba_batch_free(b);
+        return albba_fail(rc, "batch_create: %s", buf);
+    }
+    *out = b;
+    return ALBBA_OK;
+}
+
+static double inst_cost(const albba_graph *G) { return (double)G->g.nnz + (double)G->g.n; }
+
+/* Split W workgroups into teams and give every instance a team.  Fills
+ * team_first/team_size (teams), team_ptr/team_items (instances per team),
+ * wg_team (workgroups); returns the number of teams. */
+static int32_t split_teams(const albba_batch *b, int32_t W, int32_t *team_first, int32_t *team_size,
+                           int32_t *team_ptr, int32_t *team_items, int32_t *wg_team, int32_t *inst_team)
+{
+    const int32_t N = b->count;
+    double total = 0;
+    for (int32_t i = 0; i < N; ++i) total += inst_cost(b->graphs[i]);
+    int32_t T;
+    if (N <= W) {
+        T = N;
+        int32_t used = 0;
+        for (int32_t i = 0; i < N; ++i) {
+            int32_t s = (int32_t)((double)W * inst_cost(b->graphs[i]) / (total > 0 ? total : 1));
+            if (s < 1) s = 1;
+            if (s > 64) s = 64;
+            team_size[i] = s;
+            used += s;
+        }
+        while (used > W) { /* trim the largest teams */
+            int32_t j = 0;
+            for (int32_t i = 1; i < N; ++i)
+                if (team_size[i] > team_size[j]) j = i;
+            --team_size[j];
+            --used;
+        }
+        for (int32_t i = 0; i < N; ++i) {
+            team_items[i] = i;
+            team_ptr[i] = i;
+            inst_team[i] = i;
+        }
+        team_ptr[N] = N;
+    } else {
+        T = W;
+        /* longest first onto the least loaded team */
+        int32_t *order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
+        double *load = (double *)albba_calloc((size_t)T, sizeof(double));
+        int32_t *cnt = (int32_t *)albba_calloc((size_t)T + 1, sizeof(int32_t));
+        if (!order || !load || !cnt) {
+            free(order); free(load); free(cnt);
+            return -1;
+        }
+        for (int32_t i = 0; i < N; ++i) order[i] = i;
+        for (int32_t i = 1; i < N; ++i) { /* insertion sort by cost, descending (N is small) */
+            int32_t x = order[i], j = i - 1;
+            while (j >= 0 && inst_cost(b->graphs[order[j]]) < inst_cost(b->graphs[x])) {
+                order[j + 1] = order[j];
+                --j;
+            }
+            order[j + 1] = x;
+        }
+        for (int32_t k = 0; k < N; ++k) {
+            int32_t t = 0;
+            for (int32_t j = 1; j < T; ++j)
+                if (load[j] < load[t]) t = j;
+            load[t] += inst_cost(b->graphs[order[k]]);
+            inst_team[order[k]] = t;
+            ++cnt[t + 1];
+        }
+        for (int32_t t = 0; t < T; ++t) {
+            team_size[t] = 1;
+            cnt[t + 1] += cnt[t];
+            team_ptr[t] = cnt[t];
+        }
+        team_ptr[T] = cnt[T];
+        for (int32_t k = 0; k < N; ++k) team_items[cnt[inst_team[order[k]]]++] = order[k];
+        free(order); free(load); free(cnt);
+    }
+    int32_t wg = 0;
+    for (int32_t t = 0; t < T; ++t) {
+        team_first[t] = wg;
+        for (int32_t s = 0; s < team_size[t]; ++s) wg_team[wg++] = t;
+    }
+    return T;
+}
+
+int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
+                    const int32_t *mapping, int32_t workgroups, albba_batch_result *results,
+                    float *total_ms)
+{
+    if (!b || !roots) return albba_fail(ALBBA_EINVAL, "batch_run: bad argument");
+    albba_bfs_opts o;
+    if (opts) o = *opts;
+    else albba_bfs_opts_default(&o);
+    if (o.switch_policy != ALBBA_SWITCH_HEURISTIC && o.switch_policy != ALBBA_SWITCH_LEARNED)
+        return albba_fail(ALBBA_EINVAL, "batch_run: policy must be heuristic or learned");
+    if (o.switch_policy == ALBBA_SWITCH_LEARNED && !o.selector)
+        return albba_fail(ALBBA_EINVAL, "batch_run: learned policy needs a selector");
+    const int32_t N = b->count;
+    for (int32_t i = 0; i < N; ++i) {
+        if (roots[i] < 0 || roots[i] >= b->graphs[i]->n)
+            return albba_fail(ALBBA_EINVAL, "batch_run: root %d of instance %d out of range", roots[i], i);
+        if (mapping && mapping[i] != ALBBA_MAP_LANE_ROW && mapping[i] != ALBBA_MAP_WAVE_ROW)
+            return albba_fail(ALBBA_EINVAL, "batch_run: mapping %d of instance %d", mapping[i], i);
+    }
+    albd_ctx *d = b->ctx->d;
+    int cap = 0, rc = albd_batch_capacity(d, &cap);
+    if (rc) return albba_fail(rc, "batch_run: %s", albd_error());
+    int32_t W = workgroups > 0 && workgroups < cap ? workgroups : cap;
+    if (W < 1) return albba_fail(ALBBA_EHIP, "batch_run: no resident workgroup");
+    const int32_t Tmax = N < W ? N : W;
+    int32_t *team_first = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)Tmax);
+    int32_t *team_size = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)Tmax);
+    int32_t *team_ptr = (int32_t *)albba_malloc(sizeof(int32_t) * ((size_t)Tmax + 1));
+    int32_t *team_items = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
+    int32_t *wg_team = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)W);
+    int32_t *inst_team = b->team_of;
+    if (!team_first || !team_size || !team_ptr || !team_items || !wg_team) {
+        free(team_first); free(team_size); free(team_ptr); free(team_items); free(wg_team);
+        return albba_fail(ALBBA_ENOMEM, "batch_run");
+    }
+    const int32_t T = split_teams(b, W, team_first, team_size, team_ptr, team_items, wg_team, inst_team);
+    int32_t used = 0;
+    for (int32_t t = 0; t < T; ++t) used += team_size[t];
+    for (int32_t i = 0; i < N && T > 0; ++i) {
+        b->items[i].root = roots[i];
+        b->items[i].mapping = mapping ? mapping[i] : ALBD_MAP_LANE_ROW;
+        b->team_size_of[i] = team_size[inst_team[i]];
+    }
+    /* device copies of the tables */
+    void *d_items = NULL, *d_tab = NULL;
+    const size_t tab_words = (size_t)W + 3 * (size_t)T + 1 + (size_t)N;
+    if (T <= 0) rc = ALBBA_ENOMEM;
+    if (!rc) rc = albd_malloc(d, sizeof(albd_batch_item) * (size_t)N, &d_items);
+    if (!rc) rc = albd_malloc(d, sizeof(int32_t) * tab_words, &d_tab);
+    int32_t *h_tab = rc ? NULL : (int32_t *)albba_malloc(sizeof(int32_t) * tab_words);
+    if (!rc && !h_tab) rc = ALBBA_ENOMEM;
+    float ms = 0;
+    if (!rc) {
+        int32_t *p = h_tab;
+        memcpy(p, wg_team, sizeof(int32_t) * (size_t)used);
+        p += W;
+        memcpy(p, team_first, sizeof(int32_t) * (size_t)T);
+        p += T;
+        memcpy(p, team_size, sizeof(int32_t) * (size_t)T);
+        p += T;
+        memcpy(p, team_ptr, sizeof(int32_t) * ((size_t)T + 1));
+        p += T + 1;
+        memcpy(p, team_items, sizeof(int32_t) * (size_t)N);
+        albd_batch_args a;
+        memset(&a, 0, sizeof a);
+        const int32_t *dt = (const int32_t *)d_tab;
+        a.items = (const albd_batch_item *)d_items;
+        a.wg_team = dt;
+        a.team_first = dt + W;
+        a.team_size = dt + W + T;
+        a.team_ptr = dt + W + 2 * T;
+        a.team_items = dt + W + 3 * T + 1;
+        a.params = o.switch_policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(o.selector) : NULL;
+        a.spin_limit = 1u << 24;
+        if ((rc = albd_h2d(d, d_items, b->items, sizeof(albd_batch_item) * (size_t)N)) ||
+            (rc = albd_h2d(d, d_tab, h_tab, sizeof(int32_t) * tab_words)) ||
+            (rc = albd_fill_u64(d, b->ctl, 0, (size_t)N * CTL_WORDS)) || (rc = albd_event_record(d, 4)) ||
+            (rc = albd_batch_launch(d, &a, used)) || (rc = albd_event_record(d, 5)) ||
+            (rc = albd_event_elapsed(d, 4, 5, &ms)) ||
+            (rc = albd_d2h(d, b->hctl, b->ctl, sizeof(unsigned long long) * (size_t)N * CTL_WORDS))) {
+        }
+    }
+    char buf[256] = "";
+    if (rc) snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "out of host memory" : albd_error());
+    albd_free(d, d_items);
+    albd_free(d, d_tab);
+    free(h_tab); free(team_first); free(team_size); free(team_ptr); free(team_items); free(wg_team);
+    if (rc) return albba_fail(rc, "batch_run: %s", buf);
+    const double tpm = albd_clock_ticks_per_ms(d);
+    long long t_first = 0;
+    int any_error = 0;
+    for (int32_t i = 0; i < N; ++i) {
+        const long long *c = (const long long *)(b->hctl + (size_t)i * CTL_WORDS);
+        if (i == 0 || c[ALBD_RES_T0] < t_first) t_first = c[ALBD_RES_T0];
+    }
+    for (int32_t i = 0; i < N; ++i) {
+        const long long *c = (const long long *)(b->hctl + (size_t)i * CTL_WORDS);
+        const int err = c[ALBD_RES_ERROR] != 0 || (b->hctl[(size_t)i * CTL_WORDS + 2] & 0xffffffffull) != 0 ||
+                        c[ALBD_RES_T1] == 0;
+        any_error |= err;
+        if (results) {
+            albba_batch_result *r = &results[i];
+            memset(r, 0, sizeof *r);
+            r->error = err;
+            r->num_levels = (int32_t)c[ALBD_RES_LEVELS];
+            r->reached = c[ALBD_RES_REACHED];
+            r->edges_in_component = c[ALBD_RES_EDGES];
+            r->ms = err ? 0.0 : (double)(c[ALBD_RES_T1] - c[ALBD_RES_T0]) / tpm;
+            r->done_ms = err ? 0.0 : (double)(c[ALBD_RES_T1] - t_first) / tpm;
+            r->push_levels = (int32_t)c[ALBD_RES_PUSHES];
+            r->pull_levels = (int32_t)c[ALBD_RES_PULLS];
+            r->team_size = b->team_size_of[i];
+            r->team = inst_team[i];
+        }
+    }
+    if (total_ms) *total_ms = ms;
+    if (any_error) {
+        /* a team gave up waiting: its keys may be left set */
+        for (int32_t i = 0; i < N; ++i)
+            albd_fill_u64(d, b->items[i].key, ~0ull, (size_t)b->items[i].g.npos);
+        albd_sync(d);
+        return albba_fail(ALBBA_EHIP, "batch_run: a team barrier timed out");
+    }
+    return ALBBA_OK;
+}
+
+int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent)
+{
+    if (!b || i < 0 || i >= b->count) return albba_fail(ALBBA_EINVAL, "batch_results: bad instance");
+    const size_t n = (size_t)b->items[i].g.n;
+    int rc = 0;
+    if (level) rc = albd_d2h(b->ctx->d, level, b->items[i].level, 4 * n);
+    if (!rc && parent) rc = albd_d2h(b->ctx->d, parent, b->items[i].parent, 4 * n);
+    return rc ? albba_fail(rc, "batch_results: %s", albd_error()) : ALBBA_OK;
+}

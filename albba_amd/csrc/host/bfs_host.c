@@ -14,6 +14,7 @@
  */
 #include "internal.h"
 #include "../albd.h"
+#include "device_int.h"
 #include "selector_int.h"
 
 #include <math.h>
@@ -21,25 +22,6 @@
 int sigma_window = 16384;
 int perf_analysis = 0;
 int verify = 0;
-
-struct albba_ctx {
-    albd_ctx *d;
-    int device;
-};
-
-struct albba_graph {
-    albba_ctx *ctx;
-    albd_graph g;
-    albd_state st;
-    int32_t n;
-    int64_t nnz;
-    int32_t *pos_of;     /* host: vertex id -> position */
-    int64_t device_bytes;
-    void *scratch_visited; /* BEST-policy snapshots */
-    void *scratch_ctr;
-    int64_t *host_log;     /* small-frontier level sizes, 4 per level */
-    int needs_reset;       /* a BFS failed midway: pmin/pkey may be dirty */
-};
 
 /* ---- context ---- */
 

@@ -309,6 +309,45 @@ ALBBA_API int albba_bfs(albba_ctx *ctx, albba_graph *g, int32_t root,
 ALBBA_API int albba_bfs_results(albba_ctx *ctx, albba_graph *g, int32_t *level,
                                 int32_t *parent);
 
+/* ---- batched BFS ---------------------------------------------------------
+ * Many BFS instances (graphs and roots of uneven size) in one cooperative
+ * launch: the resident workgroups are split into teams, each team runs the
+ * whole level loop of its instances with the direction of every level chosen
+ * on the device.  No counterpart in the reference, which runs one BFS per
+ * process; results follow the same level/min-id-parent contract as
+ * albba_bfs. */
+typedef struct albba_batch albba_batch;
+
+enum { ALBBA_MAP_LANE_ROW = 0, ALBBA_MAP_WAVE_ROW = 1 }; /* pull: a lane or a wave per row */
+
+typedef struct albba_batch_result {
+    int32_t num_levels;
+    int32_t error;               /* nonzero: the instance did not finish       */
+    int64_t reached;             /* vertices with a level                      */
+    int64_t edges_in_component;  /* out-edges summed over the reached vertices */
+    double ms;                   /* the instance's run time on the device clock */
+    double done_ms;              /* from the first instance's start to its end */
+    int32_t push_levels;
+    int32_t pull_levels;
+    int32_t team_size;           /* workgroups that ran it                     */
+    int32_t team;                /* team index                                 */
+} albba_batch_result;
+
+/* Device scratch for one BFS on each of graphs[0..count) (a graph may appear
+ * more than once); the graphs must outlive the batch. */
+ALBBA_API int albba_batch_create(albba_ctx *ctx, albba_graph *const *graphs, int32_t count,
+                                 albba_batch **out);
+ALBBA_API void albba_batch_free(albba_batch *b);
+/* BFS from roots[i] (vertex id) on instance i, all in one launch.  opts: the
+ * heuristic (default) or learned policy; mapping[i] (NULL: lane per row)
+ * picks instance i's pull mapping; workgroups 0 uses every resident
+ * workgroup.  results[count] and total_ms (launch time) may be NULL. */
+ALBBA_API int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
+                              const int32_t *mapping, int32_t workgroups,
+                              albba_batch_result *results, float *total_ms);
+/* level/parent of instance i after albba_batch_run (either may be NULL) */
+ALBBA_API int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent);
+
 /* ---- learned direction selector ----------------------------------------- */
 #define ALBBA_MLP_IN 12
 #define ALBBA_MLP_H1 32
