@@ -103,6 +103,11 @@ class _BatchResult(C.Structure):
                 ("team", C.c_int32)]
 
 
+class _TuneConfig(C.Structure):
+    _fields_ = [("C", C.c_int32), ("sigma", C.c_int32), ("mapping", C.c_int32),
+                ("status", C.c_int32), ("ms", C.c_double), ("batch_ms", C.c_double)]
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -150,6 +155,9 @@ def lib() -> C.CDLL:
         "albba_batch_run": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
                                       C.POINTER(_BatchResult), C.POINTER(C.c_float)]),
         "albba_batch_results": (C.c_int, [C.c_void_p, C.c_int32, I32P, I32P]),
+        "albba_autotune": (C.c_int, [C.c_void_p, C.POINTER(_CSR), I32P, C.c_int32, C.POINTER(_Opts),
+                                     C.POINTER(_TuneConfig), C.c_int32, C.POINTER(_TuneConfig),
+                                     C.POINTER(_TuneConfig)]),
         "albba_selector_free": (None, [C.c_void_p]),
         "albba_selector_get_params": (C.c_int, [C.c_void_p, F32P]),
         "albba_selector_set_params": (C.c_int, [C.c_void_p, F32P]),
@@ -519,6 +527,35 @@ class Batch:
         _check(lib().albba_batch_results(self._p, int(i), lv.ctypes.data_as(I32P),
                                          pa.ctypes.data_as(I32P)))
         return lv, pa
+
+
+def autotune(ctx: Context, graph: "Graph", roots, policy: int = SWITCH_HEURISTIC,
+             selector: "Selector" = None, candidates=None) -> dict:
+    """Measure every (C, sigma) candidate (default: C in {32,64,128} x sigma in
+    {C, 256, 4096, n}) on `roots`, then the batched pull mapping on the
+    fastest layout.  Returns {"best": {...}, "results": [{...}, ...]}."""
+    r = np.ascontiguousarray(roots, dtype=np.int32)
+    o = _Opts()
+    lib().albba_bfs_opts_default(C.byref(o))
+    o.switch_policy = policy
+    if selector is not None:
+        o.selector = selector._p
+    cand = None
+    n = 12
+    if candidates is not None:
+        n = len(candidates)
+        cand = (_TuneConfig * n)()
+        for i, (c_, s_) in enumerate(candidates):
+            cand[i].C, cand[i].sigma = int(c_), int(s_)
+    res = (_TuneConfig * n)()
+    best = _TuneConfig()
+    _check(lib().albba_autotune(ctx._p, C.byref(graph._raw), r.ctypes.data_as(I32P), r.size,
+                                C.byref(o), cand, n if cand is not None else 0, res, C.byref(best)))
+
+    def as_dict(t):
+        return {f: getattr(t, f) for f, _ in _TuneConfig._fields_}
+
+    return {"best": as_dict(best), "results": [as_dict(t) for t in res]}
 
 
 class Selector:

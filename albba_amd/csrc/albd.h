@@ -211,13 +211,12 @@ ALBD_HIDDEN int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *
 /* One instance: a device graph, its own scratch, a root.  The control block
  * ctl (32 words) must be zero before the launch: [0] team barrier,
  * [1] found-list length, [2] error flag, [4..19] ring of four level counters
- * (found, out-edges, queue entries, nonzero words), [24..31] results. */
+ * (pulled vertices, out-edges, -, nonzero words), [24..31] results. */
 typedef struct albd_batch_item {
     albd_graph g;
     uint64_t *visited;           /* [nwords] */
     uint64_t *bm[3];             /* [nwords] frontier bitmaps, rotating */
-    int32_t *qa, *qb;            /* [npos] frontier queues */
-    int32_t *found;              /* [npos] vertices found by push levels, in order */
+    int32_t *found;              /* [npos] frontier queues, level after level */
     unsigned long long *key;     /* [npos] (level, parent id), all ones at rest */
     int32_t *level, *parent;     /* [n] results in vertex order */
     unsigned long long *ctl;     /* [32] see above */

@@ -16,10 +16,11 @@
 //
 // Frontier bitmaps B_L (vertices of level L) rotate over three buffers: at
 // level L the team reads B_L, sets B_{L+1} and clears B_{L-1}, which becomes
-// B_{L+2}.  A push level also lists its discoveries (found list) so that
-// their visited bits and, two levels later, their bitmap bits are handled
-// without a full pass; a pull level updates visited at once and its bitmap
-// is cleared by a full pass.
+// B_{L+2}.  A push level lists its discoveries (found list), which is the
+// next level's queue and lets the bitmap be cleared two levels later without
+// a full pass; a pulled bitmap is cleared by a full pass.  Discoverers set
+// the visited bits at once: a push needs no visited filter (the key decides),
+// a pull reads visited as the exact set of vertices found so far.
 #include "dev_common.h"
 #include "select_dev.h"
 #include "../../../include/albba.h"
@@ -38,13 +39,13 @@ struct BLds {
     uint32_t ou[kBChunk];
     int64_t scan[kBWaves];
     int32_t found[kBWin];
-    int32_t queue[kBWin];
-    unsigned int nf, nq;
-    unsigned long long bf, bq;
+    unsigned int nf;
+    unsigned long long bf;
     unsigned long long red[4][kBWaves];
     unsigned int s_cnt[kBWaves];
     unsigned long long s_base;
     int ok, choice;
+    float params[ALBBA_MLP_NPARAM];
     float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
 };
 
@@ -60,21 +61,19 @@ __device__ __forceinline__ long long *ring(const albd_batch_item &it, int L)
 
 // per-wave counts -> one atomic per workgroup into the level's ring slot
 __device__ void add_counts(BLds &S, long long *slot, unsigned long long f, unsigned long long e,
-                           unsigned long long q, unsigned long long w)
+                           unsigned long long w)
 {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     f = wave_sum(f);
     e = wave_sum(e);
-    q = wave_sum(q);
     w = wave_sum(w);
     if (lane == 0) {
         S.red[0][wid] = f;
         S.red[1][wid] = e;
-        S.red[2][wid] = q;
         S.red[3][wid] = w;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 4 && threadIdx.x != 2) {
         unsigned long long t = 0;
         for (int i = 0; i < kBWaves; ++i) t += S.red[threadIdx.x][i];
         if (t) atomicAdd(reinterpret_cast<unsigned long long *>(&slot[threadIdx.x]), t);
@@ -82,30 +81,27 @@ __device__ void add_counts(BLds &S, long long *slot, unsigned long long f, unsig
     __syncthreads();
 }
 
-// LDS found/queue lists of a push window -> the instance's global lists
-__device__ void flush_lists(BLds &S, const albd_batch_item &it, long long *slot, int32_t *qb)
+// the LDS list of a push window -> the instance's found list (one
+// reservation per window)
+__device__ void flush_found(BLds &S, const albd_batch_item &it)
 {
     __syncthreads();
-    const unsigned int nf = S.nf, nq = S.nq;
-    if (threadIdx.x == 0) {
-        S.bf = nf ? atomicAdd(&it.ctl[1], (unsigned long long)nf) : 0;
-        S.bq = nq ? atomicAdd(reinterpret_cast<unsigned long long *>(&slot[2]), (unsigned long long)nq) : 0;
-    }
+    const unsigned int nf = S.nf;
+    if (threadIdx.x == 0) S.bf = nf ? atomicAdd(&it.ctl[1], (unsigned long long)nf) : 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        S.nf = 0;
-        S.nq = 0;
-    }
-    const unsigned long long bf = S.bf, bq = S.bq;
+    if (threadIdx.x == 0) S.nf = 0;
+    const unsigned long long bf = S.bf;
     for (unsigned int i = threadIdx.x; i < nf; i += kBB) st_coherent(&it.found[bf + i], S.found[i]);
-    for (unsigned int i = threadIdx.x; i < nq; i += kBB) st_coherent(&qb[bq + i], S.queue[i]);
     __syncthreads();
 }
 
 // One BFS of one instance by a team of T workgroups (this one is `rank`).
-// Returns false when a team barrier gave up.
-__device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T,
-                             const float *params, uint32_t spin_limit)
+// The found list holds, level after level, the vertices each push found and
+// the rebuilt frontier of a push that follows a pull: the frontier queue of
+// level L is the range [fs_cur, fe_cur).  Returns false when a team barrier
+// gave up.
+__device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T, bool learned,
+                             uint32_t spin_limit)
 {
     const albd_graph &g = it.g;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
@@ -121,10 +117,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         st_coherent(&it.bm[1][i], (uint64_t)0);
         st_coherent(&it.bm[2][i], (uint64_t)0);
     }
-    if (tid == 0) {
-        S.nf = 0;
-        S.nq = 0;
-    }
+    if (tid == 0) S.nf = 0;
     if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
     const int32_t rp = g.pos_of[it.root];
     const int64_t rdeg = g.push_ptr[rp + 1] - g.push_ptr[rp];
@@ -136,33 +129,35 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         atomicOr(reinterpret_cast<unsigned long long *>(&it.bm[0][rp >> 6]), bit);
         st_coherent(&it.found[0], rp);
         st_coherent(&it.ctl[1], 1ull);
-        if (rdeg > 0) st_coherent(&it.qa[0], rp);
     }
     if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
 
     // team-uniform loop state
-    int32_t *qa = it.qa, *qb = it.qb;
-    int64_t qlen = rdeg > 0 ? 1 : 0;
-    bool have_queue = true;                 // qa holds B_L's vertices of out-degree > 0
-    unsigned long long qbase = 0;           // ctl[3] at the last barrier
     unsigned long long F = 1, mf = (unsigned long long)rdeg, nzw = 1;
     unsigned long long reached = 1, redges = (unsigned long long)rdeg;
     int prev_choice = ALBBA_CHOICE_PUSH;
-    // found-list range of B_L (pushed vertices) and of B_{L-1}; how each bitmap was made
-    int64_t fs_cur = 0, fe_cur = 1, fs_prev = 0, fe_prev = 0;
-    bool cur_by_pull = false, prev_by_pull = false;
+    int64_t fs_cur = 0, fe_cur = 1, fs_prev = 0, fe_prev = 0; // found ranges of B_L, B_{L-1}
+    bool have_queue = true;                                   // B_L's range holds its vertices
+    bool cur_by_pull = false, prev_by_pull = false;           // how B_L, B_{L-1} were made
     long long pushes = 0, pulls = 0;
+    const long long thr = (long long)(int)((double)g.n * 0.04);
     int L = 0;
     for (; F > 0; ++L) {
         uint64_t *bcur = it.bm[L % 3], *bnext = it.bm[(L + 1) % 3], *bprev = it.bm[(L + 2) % 3];
         long long *slot = ring(it, L);
-        // direction of this level (the same in every workgroup of the team)
-        int choice = heuristic_choice(F, g.n);
-        if (params) {
+        // direction of this level (the same in every workgroup of the team).
+        // A batched pull visits every word of the instance with the team's few
+        // workgroups, so the learned selector is consulted only when the
+        // frontier is large by the heuristic's count or its out-edges reach
+        // 1/16 of the unexplored edges; otherwise the level pushes.
+        int choice = (long long)F > thr ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PUSH;
+        const unsigned long long unexplored =
+            (unsigned long long)g.nnz > redges ? (unsigned long long)g.nnz - redges : 0ull;
+        if (learned && ((long long)F > thr || 16 * mf >= unexplored)) {
             if (tid == 0)
                 level_features(S.xs, g.n, g.nnz, nwords, F, mf, nzw, reached, redges, L, prev_choice);
             __syncthreads();
-            const int c = mlp_argmin(params, S.xs, S.h1, S.h2, S.ys, 0xF, &S.choice);
+            const int c = mlp_argmin(S.params, S.xs, S.h1, S.h2, S.ys, 0xF, &S.choice);
             if (c >= 0) choice = c;
         }
         const bool push = choice == ALBBA_CHOICE_PUSH;
@@ -178,65 +173,52 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                     st_coherent(&bprev[ld_coherent(&it.found[i]) >> 6], (uint64_t)0);
             }
         }
-        // pushed vertices of B_L join visited (a pulled level did it at once)
-        if (!cur_by_pull)
-            for (int64_t i = fs_cur + gtid; i < fe_cur; i += gthreads) {
-                const int32_t v = ld_coherent(&it.found[i]);
-                atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[v >> 6]), 1ull << (v & 63));
-            }
         const unsigned long long lkey = (unsigned long long)(uint32_t)(L + 1) << 32;
         unsigned long long my_f = 0, my_e = 0, my_w = 0;
         if (push) {
             ++pushes;
             if (!have_queue) {
-                // queue of B_L from its bitmap (the last level pulled); slots
-                // are reserved once per block step from ctl[3], a counter that
-                // only grows (qbase: its value at the last barrier)
+                // B_L came from a pull: list its vertices from the bitmap,
+                // one reservation per block step
                 for (int64_t wb = (int64_t)rank * kBB; wb < nwords; wb += gthreads) {
                     const int64_t w = wb + tid;
                     const uint64_t word = w < nwords ? ld_coherent(&bcur[w]) : 0ull;
-                    uint64_t keep = 0, bits = word;
-                    while (bits) {
-                        const int b = __ffsll((unsigned long long)bits) - 1;
-                        bits &= bits - 1;
-                        const int64_t p = w * 64 + b;
-                        if (g.push_ptr[p + 1] > g.push_ptr[p]) keep |= 1ull << b;
-                    }
                     unsigned long long wtot;
                     const unsigned long long ex =
-                        wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(keep), &wtot);
+                        wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(word), &wtot);
                     if (lane == 0) S.s_cnt[wid] = (unsigned int)wtot;
                     __syncthreads();
                     if (tid == 0) {
                         unsigned long long tot = 0;
                         for (int i = 0; i < kBWaves; ++i) tot += S.s_cnt[i];
-                        S.s_base = tot ? atomicAdd(&it.ctl[3], tot) : 0ull;
+                        S.s_base = tot ? atomicAdd(&it.ctl[1], tot) : 0ull;
                     }
                     __syncthreads();
-                    unsigned long long o = S.s_base - qbase + ex;
+                    unsigned long long o = S.s_base + ex;
                     for (int i = 0; i < wid; ++i) o += S.s_cnt[i];
                     __syncthreads();
-                    uint64_t kb = keep;
+                    uint64_t kb = word;
                     while (kb) {
                         const int b = __ffsll((unsigned long long)kb) - 1;
                         kb &= kb - 1;
-                        st_coherent(&qa[o++], (int32_t)(w * 64 + b));
+                        st_coherent(&it.found[o++], (int32_t)(w * 64 + b));
                     }
                 }
                 if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
-                const unsigned long long qend = ld_coherent(&it.ctl[3]);
-                qlen = (int64_t)(qend - qbase);
-                qbase = qend;
+                fs_cur = fe_cur;
+                fe_cur = (int64_t)ld_coherent(&it.ctl[1]);
             }
             // expansion: workgroup `rank` takes queue entries [q0, q1)
+            const int64_t qlen = fe_cur - fs_cur;
             const int64_t per = (qlen + T - 1) / T;
-            const int64_t q0 = (int64_t)rank * per, q1 = q0 + per < qlen ? q0 + per : qlen;
+            const int64_t q0 = fs_cur + (int64_t)rank * per;
+            const int64_t q1 = q0 + per < fe_cur ? q0 + per : fe_cur;
             for (int64_t base = q0; base < q1; base += kBChunk) { // uniform in the workgroup
                 const int cnt_e = (int)(q1 - base < kBChunk ? q1 - base : kBChunk);
                 int64_t a0 = 0, e0 = 0;
                 uint32_t o0 = 0;
                 if (tid < cnt_e) {
-                    const int32_t u = ld_coherent(&qa[base + tid]);
+                    const int32_t u = ld_coherent(&it.found[base + tid]);
                     a0 = g.push_ptr[u];
                     e0 = g.push_ptr[u + 1];
                     o0 = (uint32_t)g.row_order[u];
@@ -274,39 +256,30 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                             ++e;
                         }
                     }
-                    uint64_t vis[kBPer];
-#pragma unroll
-                    for (int i = 0; i < kBPer; ++i) vis[i] = okv[i] ? it.visited[v[i] >> 6] : ~0ull;
+                    // the key decides: no visited filter on the way
                     bool won[kBPer];
 #pragma unroll
                     for (int i = 0; i < kBPer; ++i) {
                         won[i] = false;
-                        if (!((vis[i] >> (v[i] & 63)) & 1ull)) {
-                            const unsigned long long old = atomicMin(&it.key[v[i]], lkey | ou[i]);
-                            won[i] = old == ~0ull;
-                        }
+                        if (okv[i]) won[i] = atomicMin(&it.key[v[i]], lkey | ou[i]) == ~0ull;
                     }
-                    int64_t dv[kBPer];
 #pragma unroll
                     for (int i = 0; i < kBPer; ++i) {
-                        dv[i] = 0;
                         if (won[i]) {
-                            dv[i] = g.push_ptr[v[i] + 1] - g.push_ptr[v[i]];
-                            const unsigned long long old = atomicOr(
-                                reinterpret_cast<unsigned long long *>(&bnext[v[i] >> 6]), 1ull << (v[i] & 63));
+                            const uint64_t bit = 1ull << (v[i] & 63);
+                            const unsigned long long old =
+                                atomicOr(reinterpret_cast<unsigned long long *>(&bnext[v[i] >> 6]), bit);
+                            atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[v[i] >> 6]), bit);
                             my_w += old == 0 ? 1 : 0;
-                            my_f += 1;
-                            my_e += (unsigned long long)dv[i];
+                            my_e += (unsigned long long)(g.push_ptr[v[i] + 1] - g.push_ptr[v[i]]);
                         }
                     }
 #pragma unroll
                     for (int i = 0; i < kBPer; ++i) {
                         const unsigned int pf = wave_append(&S.nf, won[i]);
                         if (won[i]) S.found[pf] = v[i];
-                        const unsigned int pq = wave_append(&S.nq, dv[i] > 0);
-                        if (dv[i] > 0) S.queue[pq] = v[i];
                     }
-                    flush_lists(S, it, slot, qb);
+                    flush_found(S, it);
                 }
                 __syncthreads(); // pre/start/ou are restaged
             }
@@ -320,8 +293,8 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
             for (int64_t w = w0 + wid; w < w1; w += kBWaves) {
                 const uint64_t hl = g.haslen[w];
                 if (hl == 0) continue;
-                const uint64_t seen = ld_coherent(&it.visited[w]) | ld_coherent(&bcur[w]);
-                const uint64_t unres = hl & ~seen;
+                // every vertex found so far has its visited bit (set by its discoverer)
+                const uint64_t unres = hl & ~ld_coherent(&it.visited[w]);
                 if (unres == 0) continue;
                 const int64_t p = w * 64 + lane;
                 int32_t par = -1;
@@ -361,10 +334,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                     }
                 }
                 bool won = false;
-                if (par >= 0) {
-                    const unsigned long long old = atomicMin(&it.key[p], lkey | (uint32_t)par);
-                    won = old == ~0ull;
-                }
+                if (par >= 0) won = atomicMin(&it.key[p], lkey | (uint32_t)par) == ~0ull;
                 const uint64_t fm = __ballot(won);
                 if (fm) {
                     if (lane == 0) {
@@ -380,31 +350,27 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                 }
             }
         }
-        add_counts(S, slot, my_f, my_e, 0, my_w);
+        add_counts(S, slot, my_f, my_e, my_w);
         if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
-        // B_{L+1} is complete: its size, out-edges, words and (push) queue
-        F = (unsigned long long)ld_coherent(&slot[0]);
+        // B_{L+1} is complete: its size, out-edges, words and (push) list
         mf = (unsigned long long)ld_coherent(&slot[1]);
         nzw = (unsigned long long)ld_coherent(&slot[3]);
-        reached += F;
-        redges += mf;
         fs_prev = fs_cur;
         fe_prev = fe_cur;
         prev_by_pull = cur_by_pull;
         if (push) {
+            const int64_t total = (int64_t)ld_coherent(&it.ctl[1]);
+            F = (unsigned long long)(total - fe_cur);
             fs_cur = fe_cur;
-            fe_cur = (int64_t)ld_coherent(&it.ctl[1]);
-            qlen = ld_coherent(&slot[2]);
-            int32_t *t = qa;
-            qa = qb;
-            qb = t;
-            have_queue = true;
-            cur_by_pull = false;
+            fe_cur = total;
         } else {
-            fs_cur = fe_cur; // no found entries for a pulled level
-            have_queue = false;
-            cur_by_pull = true;
+            F = (unsigned long long)ld_coherent(&slot[0]);
+            fs_cur = fe_cur; // a pulled level lists nothing
         }
+        have_queue = push;
+        cur_by_pull = !push;
+        reached += F;
+        redges += mf;
         prev_choice = push ? ALBBA_CHOICE_PUSH : ALBBA_CHOICE_PULL;
     }
     // results in vertex order from the keys; the keys go back to all ones
@@ -429,14 +395,17 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
     return true;
 }
 
-__global__ __launch_bounds__(kBB) void k_bfs_batch(albd_batch_args a)
+__global__ __launch_bounds__(kBB, 4) void k_bfs_batch(albd_batch_args a)
 {
     __shared__ BLds S;
+    if (a.params)
+        for (int i = threadIdx.x; i < ALBBA_MLP_NPARAM; i += kBB) S.params[i] = a.params[i];
+    __syncthreads();
     const int t = a.wg_team[blockIdx.x];
     const int rank = (int)blockIdx.x - a.team_first[t], T = a.team_size[t];
     for (int j = a.team_ptr[t]; j < a.team_ptr[t + 1]; ++j) {
         const albd_batch_item &it = a.items[a.team_items[j]];
-        if (!run_instance(S, it, rank, T, a.params, a.spin_limit)) {
+        if (!run_instance(S, it, rank, T, a.params != nullptr, a.spin_limit)) {
             if (threadIdx.x == 0)
                 st_coherent(reinterpret_cast<long long *>(it.ctl) + ALBD_RES_ERROR, 1ll);
             break;

@@ -3,11 +3,10 @@
  * resident workgroups into teams, one cooperative launch, per-instance
  * results from the device clock.
  *
- * Team split: with at most as many instances as workgroups every instance
- * gets its own team, sized in proportion to its work (stored entries plus
- * vertices), at least one workgroup; with more instances than workgroups,
- * one-workgroup teams take the instances longest-first, each to the least
- * loaded team.
+ * Team split (split_teams): an instance's work is its stored entries plus
+ * its vertices; instances worth at least two workgroups of the fair share get
+ * a team of their own, the others share one-workgroup teams, largest first
+ * onto the least loaded team.
  */
 #include "internal.h"
 #include "device_int.h"
@@ -80,8 +79,6 @@ int albba_batch_create(albba_ctx *ctx, albba_graph *const *graphs, int32_t count
         it->g = G->g;
         it->visited = (uint64_t *)bdalloc(b, 8 * nw, &rc);
         for (int k = 0; k < 3; ++k) it->bm[k] = (uint64_t *)bdalloc(b, 8 * nw, &rc);
-        it->qa = (int32_t *)bdalloc(b, 4 * np, &rc);
-        it->qb = (int32_t *)bdalloc(b, 4 * np, &rc);
         it->found = (int32_t *)bdalloc(b, 4 * np, &rc);
         it->key = (unsigned long long *)bdalloc(b, 8 * np, &rc);
         it->level = (int32_t *)bdalloc(b, 4 * n, &rc);
@@ -105,78 +102,84 @@ static double inst_cost(const albba_graph *G) { return (double)G->g.nnz + (doubl
 
 /* Split W workgroups into teams and give every instance a team.  Fills
  * team_first/team_size (teams), team_ptr/team_items (instances per team),
- * wg_team (workgroups); returns the number of teams. */
+ * wg_team (workgroups); returns the number of teams.
+ *
+ * An instance's fair share is W * cost / total.  Instances whose share is at
+ * least 2 workgroups get a team of their own of that size (at most 64, and
+ * at most half of W together); the other instances go, largest first, to the
+ * least loaded of the remaining one-workgroup teams. */
 static int32_t split_teams(const albba_batch *b, int32_t W, int32_t *team_first, int32_t *team_size,
                            int32_t *team_ptr, int32_t *team_items, int32_t *wg_team, int32_t *inst_team)
 {
     const int32_t N = b->count;
     double total = 0;
     for (int32_t i = 0; i < N; ++i) total += inst_cost(b->graphs[i]);
-    int32_t T;
-    if (N <= W) {
-        T = N;
-        int32_t used = 0;
-        for (int32_t i = 0; i < N; ++i) {
-            int32_t s = (int32_t)((double)W * inst_cost(b->graphs[i]) / (total > 0 ? total : 1));
-            if (s < 1) s = 1;
-            if (s > 64) s = 64;
-            team_size[i] = s;
-            used += s;
+    if (total <= 0) total = 1;
+    int32_t *order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
+    int32_t *share = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
+    double *load = (double *)albba_calloc((size_t)W, sizeof(double));
+    int32_t *cnt = (int32_t *)albba_calloc((size_t)W + 1, sizeof(int32_t));
+    if (!order || !share || !load || !cnt) {
+        free(order); free(share); free(load); free(cnt);
+        return -1;
+    }
+    for (int32_t i = 0; i < N; ++i) {
+        order[i] = i;
+        double s = (double)W * inst_cost(b->graphs[i]) / total;
+        share[i] = s >= 64 ? 64 : (int32_t)s;
+    }
+    for (int32_t i = 1; i < N; ++i) { /* insertion sort by cost, descending */
+        int32_t x = order[i], j = i - 1;
+        while (j >= 0 && inst_cost(b->graphs[order[j]]) < inst_cost(b->graphs[x])) {
+            order[j + 1] = order[j];
+            --j;
         }
-        while (used > W) { /* trim the largest teams */
-            int32_t j = 0;
-            for (int32_t i = 1; i < N; ++i)
-                if (team_size[i] > team_size[j]) j = i;
-            --team_size[j];
-            --used;
-        }
-        for (int32_t i = 0; i < N; ++i) {
-            team_items[i] = i;
-            team_ptr[i] = i;
-            inst_team[i] = i;
-        }
-        team_ptr[N] = N;
-    } else {
-        T = W;
-        /* longest first onto the least loaded team */
-        int32_t *order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
-        double *load = (double *)albba_calloc((size_t)T, sizeof(double));
-        int32_t *cnt = (int32_t *)albba_calloc((size_t)T + 1, sizeof(int32_t));
-        if (!order || !load || !cnt) {
-            free(order); free(load); free(cnt);
-            return -1;
-        }
-        for (int32_t i = 0; i < N; ++i) order[i] = i;
-        for (int32_t i = 1; i < N; ++i) { /* insertion sort by cost, descending (N is small) */
-            int32_t x = order[i], j = i - 1;
-            while (j >= 0 && inst_cost(b->graphs[order[j]]) < inst_cost(b->graphs[x])) {
-                order[j + 1] = order[j];
-                --j;
-            }
-            order[j + 1] = x;
-        }
-        for (int32_t k = 0; k < N; ++k) {
-            int32_t t = 0;
-            for (int32_t j = 1; j < T; ++j)
-                if (load[j] < load[t]) t = j;
-            load[t] += inst_cost(b->graphs[order[k]]);
-            inst_team[order[k]] = t;
-            ++cnt[t + 1];
-        }
-        for (int32_t t = 0; t < T; ++t) {
-            team_size[t] = 1;
-            cnt[t + 1] += cnt[t];
-            team_ptr[t] = cnt[t];
-        }
-        team_ptr[T] = cnt[T];
-        for (int32_t k = 0; k < N; ++k) team_items[cnt[inst_team[order[k]]]++] = order[k];
-        free(order); free(load); free(cnt);
+        order[j + 1] = x;
+    }
+    /* own teams for the big instances, within half of the workgroups and
+     * leaving one workgroup for every remaining instance when N is small */
+    int32_t T = 0, used = 0, nbig = 0;
+    for (int32_t k = 0; k < N; ++k) {
+        const int32_t i = order[k];
+        const int32_t rest = N - k - 1;          /* instances still to place */
+        int32_t s = share[i];
+        if (s < 2) break;
+        const int32_t room = W - used - (rest < W / 2 ? rest : W / 2);
+        if (s > room) s = room;
+        if (s < 2 || used + s > (N <= W ? W : W / 2)) break;
+        team_size[T] = s;
+        inst_team[i] = T;
+        cnt[T + 1] = 1;
+        ++T;
+        used += s;
+        ++nbig;
+    }
+    /* one-workgroup teams for the rest, largest instance first */
+    const int32_t small_teams = W - used < N - nbig ? W - used : N - nbig;
+    const int32_t T0 = T;
+    for (int32_t t = 0; t < small_teams; ++t) team_size[T0 + t] = 1;
+    T += small_teams;
+    for (int32_t k = nbig; k < N; ++k) {
+        const int32_t i = order[k];
+        int32_t t = T0;
+        for (int32_t j = T0 + 1; j < T; ++j)
+            if (load[j] < load[t]) t = j;
+        load[t] += inst_cost(b->graphs[i]);
+        inst_team[i] = t;
+        ++cnt[t + 1];
+    }
+    for (int32_t t = 0; t < T; ++t) cnt[t + 1] += cnt[t];
+    for (int32_t t = 0; t <= T; ++t) team_ptr[t] = cnt[t];
+    for (int32_t k = 0; k < N; ++k) {
+        const int32_t i = order[k];
+        team_items[cnt[inst_team[i]]++] = i;
     }
     int32_t wg = 0;
     for (int32_t t = 0; t < T; ++t) {
         team_first[t] = wg;
         for (int32_t s = 0; s < team_size[t]; ++s) wg_team[wg++] = t;
     }
+    free(order); free(share); free(load); free(cnt);
     return T;
 }
 

@@ -135,6 +135,137 @@ def cpu_baseline(g, roots, gpu_levels):
                       f"harmonic mean", "levels_match_gpu": parity}
 
 
+def batch_specs(count, mix_seed):
+    """Seeded instance mix: RMAT s14-20, ef 8-32, a 0.45-0.65 (b = c =
+    (1 - a - 0.05) / 2); Erdos-Renyi n = 2^16..2^20, mean degree 4-16; square
+    4-neighbour grids of side 256..2048 with 10% of the edges removed."""
+    import albba_amd as A
+
+    def u01(x):
+        return (x >> 11) / float(1 << 53)
+
+    specs = []
+    for k in range(count):
+        r = [A.rng_u64(mix_seed, 16 * k + i) for i in range(8)]
+        kind = r[0] % 3
+        if kind == 0:
+            specs.append(("rmat", dict(scale=14 + r[1] % 7, edgefactor=8 + r[2] % 25,
+                                       a=round(0.45 + 0.2 * u01(r[3]), 4),
+                                       seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
+        elif kind == 1:
+            specs.append(("er", dict(n=1 << (16 + r[1] % 5), mean_degree=round(4 + 12 * u01(r[2]), 3),
+                                     seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
+        else:
+            side = 256 + r[1] % 1793
+            specs.append(("grid", dict(rows=side, cols=side, p_remove=0.1,
+                                       seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
+    return specs
+
+
+def make_graph(kind, p):
+    import albba_amd as A
+
+    if kind == "rmat":
+        b = (1.0 - p["a"] - 0.05) / 2.0
+        return A.Graph.rmat(p["scale"], p["edgefactor"], p["a"], b, b, seed=p["seed"],
+                            perm_seed=p["perm_seed"])
+    if kind == "er":
+        return A.Graph.er(p["n"], p["mean_degree"], seed=p["seed"], perm_seed=p["perm_seed"])
+    return A.Graph.grid(p["rows"], p["cols"], p["p_remove"], seed=p["seed"], perm_seed=p["perm_seed"])
+
+
+def run_batch_workload(args):
+    """Batched multi-instance BFS: every instance in one cooperative launch.
+    Sampled instances are checked bit for bit against the single-BFS path."""
+    import albba_amd as A
+
+    count, mix_seed = args.instances, 6
+    specs = batch_specs(count, mix_seed)
+    ctx = A.Context(0)
+    sel = None
+    if os.path.exists(SELECTOR):
+        sel = A.Selector(ctx)
+        sel.load(SELECTOR)
+    # the selector is trained on RMAT levels; the batch defaults to the heuristic
+    policy = A.SWITCH_LEARNED if sel is not None and args.policy == "learned" else A.SWITCH_HEURISTIC
+    if policy == A.SWITCH_HEURISTIC:
+        sel = None
+    # layout per graph family from the autotuner, on the family's smallest instance
+    t = time.time()
+    tuned = {}
+    for kind in ("rmat", "er", "grid"):
+        idx = [i for i, (k, _) in enumerate(specs) if k == kind]
+        if not idx:
+            continue
+        small = min(idx, key=lambda i: (specs[i][1].get("scale", 0), specs[i][1].get("n", 0),
+                                        specs[i][1].get("rows", 0)))
+        g = make_graph(*specs[small])
+        roots = g.pick_roots(4, 5, largest_cc=(kind == "grid"))
+        tuned[kind] = A.autotune(ctx, g, roots, policy=policy, selector=sel)["best"]
+        log(f"autotune {kind}: C={tuned[kind]['C']} sigma={tuned[kind]['sigma']} "
+            f"mapping={tuned[kind]['mapping']} ({tuned[kind]['ms']:.3f} ms/BFS)")
+    log(f"autotuning {time.time() - t:.1f}s")
+    t = time.time()
+    dgs, roots, nnz, kinds, keep = [], [], [], [], {}
+    for i, (kind, p) in enumerate(specs):
+        g = make_graph(kind, p)
+        cfg = tuned[kind]
+        sigma = cfg["sigma"] if cfg["sigma"] in (cfg["C"], 256, 4096) else max(g.n, cfg["C"])
+        sell = g.sellcs(cfg["C"], sigma, 1)
+        dgs.append(A.DeviceGraph(ctx, sell))
+        del sell
+        r = g.pick_roots(1, 1000 + i, largest_cc=(kind == "grid"))
+        roots.append(int(r[0]))
+        nnz.append(g.nnz)
+        kinds.append(kind)
+        if i < 3 or i % 64 == 0:
+            keep[i] = dgs[-1]  # checked against the single-BFS path below
+        if (i + 1) % 32 == 0:
+            log(f"  {i + 1}/{count} instances built ({time.time() - t:.1f}s)")
+    mapping = [tuned[k]["mapping"] for k in kinds]
+    b = A.Batch(ctx, dgs)
+    out = b.run(roots, policy=policy, selector=sel, mapping=mapping)  # warm-up
+    exact = True
+    for i, dg in keep.items():
+        lv, pa = b.results(i)
+        o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
+        exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
+    del keep
+    log(f"warm-up done, sampled instances bit-exact: {exact}")
+    runs = []
+    for s in range(args.steps):
+        out = b.run(roots, policy=policy, selector=sel, mapping=mapping)
+        runs.append(out)
+        log(f"  step {s}: {out['total_ms']:.2f} ms")
+    ms = float(np.mean([o["total_ms"] for o in runs]))
+    total = float(np.sum(nnz))
+    done = np.concatenate([o["done_ms"] for o in runs])
+    serv = np.concatenate([o["ms"] for o in runs])
+    line = {
+        "metric": f"batched BFS total GTEPS, {count} instances (RMAT s14-20, ER 2^16-2^20, "
+                  "grids 256^2-2048^2), one cooperative launch",
+        "value": round(total / (ms * 1e6), 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded mix (splitmix64 counter RNG), one root per instance",
+        "config": {"workload": f"batch{count}_mix{mix_seed}", "instances": count,
+                   "rmat": kinds.count("rmat"), "er": kinds.count("er"), "grid": kinds.count("grid"),
+                   "nnz_stored_total": int(total), "policy": "learned" if sel else "heuristic",
+                   "layouts": tuned},
+        "latency_ms": {"p50": round(float(np.percentile(done, 50)), 3),
+                       "p99": round(float(np.percentile(done, 99)), 3),
+                       "service_p50": round(float(np.percentile(serv, 50)), 3),
+                       "service_p99": round(float(np.percentile(serv, 99)), 3)},
+        "levels_max": int(max(o["num_levels"].max() for o in runs)),
+        "sampled_bit_exact": exact,
+    }
+    print(json.dumps(line), flush=True)
+    b.close()
+    for dg in dgs:
+        dg.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +274,9 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--roots", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["rmat24", "batch"], default="rmat24",
+                    help="rmat24: the headline metric; batch: the batched multi-instance BFS")
+    ap.add_argument("--instances", type=int, default=256, help="batch workload size")
     ap.add_argument("--policy", choices=["both", "heuristic", "learned"], default="both",
                     help="run one policy only (profiling); 'both' checks bit-exactness")
     args = ap.parse_args()
@@ -151,6 +285,9 @@ def main():
               "(a single BFS would shard as a 1D slice partition, see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        run_batch_workload(args)
+        return
 
     import albba_amd as A
 

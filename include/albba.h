@@ -348,6 +348,29 @@ ALBBA_API int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_
 /* level/parent of instance i after albba_batch_run (either may be NULL) */
 ALBBA_API int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent);
 
+/* ---- autotuner -------------------------------------------------------------
+ * Picks the layout of one graph by measurement: every (C, sigma) candidate is
+ * built, uploaded and timed on the sample roots through albba_bfs with the
+ * given options (mean BFS time after one warm-up run), then both pull
+ * mappings of the batched BFS are timed on the fastest layout (one instance
+ * per sample root).  The reference fixes C = 256 and takes sigma from -w
+ * (bfs/src/bfs_SELLCSv2.c:161-163); it has no tuner. */
+typedef struct albba_tune_config {
+    int32_t C;          /* slice height                                  */
+    int32_t sigma;      /* sorting window (>= n: one global window)      */
+    int32_t mapping;    /* batched pull: ALBBA_MAP_*                     */
+    int32_t status;     /* 0, or the error code of this candidate       */
+    double ms;          /* mean BFS time over the sample roots           */
+    double batch_ms;    /* batched launch time with this mapping         */
+} albba_tune_config;
+
+/* candidates == NULL: C in {32, 64, 128} x sigma in {C, 256, 4096, n}
+ * (ncand ignored, 12 results).  results[ncand or 12] may be NULL. */
+ALBBA_API int albba_autotune(albba_ctx *ctx, const albba_csr *g, const int32_t *roots,
+                             int32_t nroots, const albba_bfs_opts *opts,
+                             const albba_tune_config *candidates, int32_t ncand,
+                             albba_tune_config *results, albba_tune_config *best);
+
 /* ---- learned direction selector ----------------------------------------- */
 #define ALBBA_MLP_IN 12
 #define ALBBA_MLP_H1 32
