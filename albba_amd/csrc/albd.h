@@ -109,8 +109,7 @@ typedef struct albd_state {
     int64_t *prefix;            /* [npos+1]  degree prefix of the queue */
     int64_t *block_sums;        /* [ALBD_SCAN_BLOCKS+1] */
     uint32_t *pmin;             /* [npos]    push: min parent id per position */
-    int32_t *level_pos;         /* [npos]    BFS level by position (-1: unreached) */
-    int32_t *parent_pos;        /* [npos]    parent vertex id by position */
+    unsigned long long *lp;     /* [npos]    level << 32 | parent vertex id, by position */
     int32_t *level;             /* [n]       vertex-id space, written at the end */
     int32_t *parent;            /* [n]       */
     int32_t *active;            /* [nwords]  unvisited-word list */
@@ -158,7 +157,7 @@ ALBD_HIDDEN int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *ha
 /* reset every state array and seed the root (position rp, vertex id ro) */
 ALBD_HIDDEN int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st,
                               int32_t rp, int32_t ro);
-/* level/parent from position order (level_pos, parent_pos) to vertex order */
+/* level/parent from position order (st->lp) to vertex order */
 ALBD_HIDDEN int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st);
 /* zero st->next and the per-level slot counters */
 ALBD_HIDDEN int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_state *st);

@@ -236,8 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
                 }
                 if ((found >> lane) & 1ull) {
                     // position order: the lanes of a word write one run
-                    a.st.level_pos[p] = a.level + 1;
-                    a.st.parent_pos[p] = g.row_order[parent_pos];
+                    a.st.lp[p] = pack_lp(a.level + 1, g.row_order[parent_pos]);
                     edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
                 }
                 cnt += __popcll(found);
@@ -289,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint6
 __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, int64_t npos,
                                                       int32_t nwords)
 {
-    // level_pos/parent_pos hold stale values outside the visited set (the
+    // lp holds stale values outside the visited set (the
     // vertex-order pass masks them), and pmin is all ones at rest (the push
     // finalize restores every entry it consumed), so only the bitmaps reset
     (void)n;
@@ -312,8 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, 
 
 __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 {
-    st.level_pos[rp] = 0;
-    st.parent_pos[rp] = ro;
+    st.lp[rp] = pack_lp(0, ro);
     const uint64_t bit = 1ull << (rp & 63);
     st.visited[rp >> 6] = bit;
     st.front[rp >> 6] = bit;
@@ -339,9 +337,13 @@ constexpr int kSelectBlock = 256;
 __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel)
 {
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
+    __shared__ float s_params[ALBBA_MLP_NPARAM];
     __shared__ unsigned long long s_red[6][kSelectBlock / 64];
     __shared__ int s_choice;
     const int lane = lane_id(), wid = threadIdx.x >> 6, tid = threadIdx.x;
+    // the weights travel to LDS while the slots are summed
+    if (sel.params)
+        for (int i = tid; i < ALBBA_MLP_NPARAM; i += kSelectBlock) s_params[i] = sel.params[i];
     unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
     for (int i = tid; i < ALBD_NSLOT; i += kSelectBlock) {
         const albd_slot &sl = st.slots[i];
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         level_features(xs, g.n, g.nnz, g.nwords, F, mf, nzw, reached, redges, sel.level,
                        sel.prev_choice);
     __syncthreads();
-    const int choice = sel.params ? mlp_argmin(sel.params, xs, h1, h2, ys, sel.valid_mask, &s_choice) : -1;
+    const int choice = sel.params ? mlp_argmin(s_params, xs, h1, h2, ys, sel.valid_mask, &s_choice) : -1;
     if (tid == 0) {
         const int heur = heuristic_choice(F, g.n);
         st.ctr->frontier = F;
@@ -510,8 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
                 const int64_t d = g.push_ptr[pp + 1] - g.push_ptr[pp];
                 keep = d > 0;
                 if (FINALIZE) {
-                    st.level_pos[pp] = level + 1;
-                    st.parent_pos[pp] = (int32_t)st.pmin[pp];
+                    st.lp[pp] = pack_lp(level + 1, (int32_t)st.pmin[pp]);
                     st.pmin[pp] = 0xFFFFFFFFu;
                     edges += (unsigned long long)d;
                 }
@@ -542,15 +543,34 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
 }
 
 // level and parent from position order to vertex order, once per BFS: the
-// writes are coalesced, the reads gather within a sigma window
+// writes are coalesced, the reads gather within a sigma window; four
+// vertices per thread keep four gathers in flight
 __global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st)
 {
-    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < g.n;
-         v += (int64_t)gridDim.x * kBlock) {
-        const int32_t p = g.pos_of[v];
-        const bool reached = (st.visited[p >> 6] >> (p & 63)) & 1ull;
-        st.level[v] = reached ? st.level_pos[p] : -1;
-        st.parent[v] = reached ? st.parent_pos[p] : -1;
+    constexpr int kV = 4;
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    for (int64_t v0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; v0 < g.n; v0 += step * kV) {
+        int32_t p[kV];
+#pragma unroll
+        for (int k = 0; k < kV; ++k) {
+            const int64_t v = v0 + k * step;
+            p[k] = v < g.n ? g.pos_of[v] : 0;
+        }
+        uint64_t vis[kV];
+        unsigned long long x[kV];
+#pragma unroll
+        for (int k = 0; k < kV; ++k) {
+            vis[k] = st.visited[p[k] >> 6];
+            x[k] = st.lp[p[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kV; ++k) {
+            const int64_t v = v0 + k * step;
+            if (v >= g.n) break;
+            const bool reached = (vis[k] >> (p[k] & 63)) & 1ull;
+            st.level[v] = reached ? (int32_t)(x[k] >> 32) : -1;
+            st.parent[v] = reached ? (int32_t)(uint32_t)x[k] : -1;
+        }
     }
 }
 
@@ -788,8 +808,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
         for (int64_t i = fo_prev + gtid; i < fo; i += gthreads) {
             const int32_t v = ld_coherent(&found[i]);
             const unsigned long long key = ld_coherent(&st.pkey[v]);
-            st.level_pos[v] = (int32_t)(key >> 32);
-            st.parent_pos[v] = (int32_t)(uint32_t)key;
+            st.lp[v] = key; // the same (level, parent) packing
             atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), 1ull << (v & 63));
         }
         // (2) expansion: workgroup w takes queue entries [q0, q1)
@@ -912,8 +931,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
             for (int64_t i = fo + gtid; i < total; i += gthreads) {
                 const int32_t v = ld_coherent(&found[i]);
                 const unsigned long long key = ld_coherent(&st.pkey[v]);
-                st.level_pos[v] = (int32_t)(key >> 32);
-                st.parent_pos[v] = (int32_t)(uint32_t)key;
+                st.lp[v] = key;
                 const uint64_t bit = 1ull << (v & 63);
                 atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), bit);
                 const unsigned long long old =

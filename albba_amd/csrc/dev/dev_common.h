@@ -17,6 +17,12 @@ void albd_set_coop_off(albd_ctx *c);
         if (e_ != hipSuccess) return albd_set_error(e_, #x, __FILE__, __LINE__); \
     } while (0)
 
+// (level, parent vertex id) of one position, as one 8-byte store
+__device__ __forceinline__ unsigned long long pack_lp(int32_t level, int32_t parent)
+{
+    return ((unsigned long long)(uint32_t)level << 32) | (unsigned long long)(uint32_t)parent;
+}
+
 // wave64 helpers (CDNA wavefronts are 64 lanes wide)
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 

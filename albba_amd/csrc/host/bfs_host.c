@@ -143,8 +143,7 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, (void *)G->g.push_col);
     albd_free(d, (void *)G->g.haslen);
     albd_free(d, (void *)G->g.pos_of);
-    albd_free(d, G->st.level_pos);
-    albd_free(d, G->st.parent_pos);
+    albd_free(d, G->st.lp);
     albd_free(d, G->st.visited);
     albd_free(d, G->st.front);
     albd_free(d, G->st.next);
@@ -223,8 +222,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     st->prefix = (int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
     st->block_sums = (int64_t *)dalloc(G, sizeof(int64_t) * (ALBD_SCAN_BLOCKS + 1), &rc);
     st->pmin = (uint32_t *)dalloc(G, sizeof(uint32_t) * (size_t)npos, &rc);
-    st->level_pos = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
-    st->parent_pos = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->lp = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
     st->level = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
     st->parent = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
     st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
