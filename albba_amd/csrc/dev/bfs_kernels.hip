@@ -105,15 +105,17 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
             // a slice are sorted by length, so column j holds the lanes whose
             // row is longer than j and its size is a ballot away
             int s, l, len, len_o = 0;
+            // row lengths stream like the columns: non-temporal, so that the
+            // frontier and visited bitmaps keep their L2 lines
             if constexpr (C <= 64) {
                 s = (int)(p / C);
                 l = (int)(p % C);
-                len = g.row_len[p];
+                len = __builtin_nontemporal_load(&g.row_len[p]);
             } else {
                 s = w >> 1;
                 l = ((w & 1) << 6) + lane;
-                len = g.row_len[p];
-                len_o = g.row_len[(int64_t)s * 128 + (((w & 1) ^ 1) << 6) + lane];
+                len = __builtin_nontemporal_load(&g.row_len[p]);
+                len_o = __builtin_nontemporal_load(&g.row_len[(int64_t)s * 128 + (((w & 1) ^ 1) << 6) + lane]);
             }
             const uint64_t vis = a.st.visited[w];
             int64_t cp = g.slice_ptr[s];
@@ -542,34 +544,38 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
     }
 }
 
-// level and parent from position order to vertex order, once per BFS: the
-// writes are coalesced, the reads gather within a sigma window; four
-// vertices per thread keep four gathers in flight
+// level and parent from position order to vertex order, once per BFS.  A
+// block takes 4096 consecutive vertex ids: with sigma windows of 4096 rows
+// their positions form one window, so the gathers stay inside one 32 KB
+// stretch of lp read through one XCD's L2, and the writes are coalesced.
+constexpr int kVertexChunk = 4096;
+
 __global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st)
 {
-    constexpr int kV = 4;
-    const int64_t step = (int64_t)gridDim.x * kBlock;
-    for (int64_t v0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; v0 < g.n; v0 += step * kV) {
-        int32_t p[kV];
+    constexpr int kV = 4; // vertices in flight per thread
+    for (int64_t c0 = (int64_t)blockIdx.x * kVertexChunk; c0 < g.n; c0 += (int64_t)gridDim.x * kVertexChunk) {
+        for (int64_t v0 = c0 + threadIdx.x; v0 < c0 + kVertexChunk && v0 < g.n; v0 += kBlock * kV) {
+            int32_t p[kV];
 #pragma unroll
-        for (int k = 0; k < kV; ++k) {
-            const int64_t v = v0 + k * step;
-            p[k] = v < g.n ? g.pos_of[v] : 0;
-        }
-        uint64_t vis[kV];
-        unsigned long long x[kV];
+            for (int k = 0; k < kV; ++k) {
+                const int64_t v = v0 + k * kBlock;
+                p[k] = (v < g.n && v < c0 + kVertexChunk) ? g.pos_of[v] : 0;
+            }
+            uint64_t vis[kV];
+            unsigned long long x[kV];
 #pragma unroll
-        for (int k = 0; k < kV; ++k) {
-            vis[k] = st.visited[p[k] >> 6];
-            x[k] = st.lp[p[k]];
-        }
+            for (int k = 0; k < kV; ++k) {
+                vis[k] = st.visited[p[k] >> 6];
+                x[k] = st.lp[p[k]];
+            }
 #pragma unroll
-        for (int k = 0; k < kV; ++k) {
-            const int64_t v = v0 + k * step;
-            if (v >= g.n) break;
-            const bool reached = (vis[k] >> (p[k] & 63)) & 1ull;
-            st.level[v] = reached ? (int32_t)(x[k] >> 32) : -1;
-            st.parent[v] = reached ? (int32_t)(uint32_t)x[k] : -1;
+            for (int k = 0; k < kV; ++k) {
+                const int64_t v = v0 + k * kBlock;
+                if (v >= g.n || v >= c0 + kVertexChunk) break;
+                const bool reached = (vis[k] >> (p[k] & 63)) & 1ull;
+                st.level[v] = reached ? (int32_t)(x[k] >> 32) : -1;
+                st.parent[v] = reached ? (int32_t)(uint32_t)x[k] : -1;
+            }
         }
     }
 }
@@ -1039,8 +1045,8 @@ int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_
 
 int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
-    int64_t blocks = ((int64_t)g->n + kBlock - 1) / kBlock;
-    int64_t cap = (int64_t)albd_num_cus(c) * 16;
+    int64_t blocks = ((int64_t)g->n + kVertexChunk - 1) / kVertexChunk;
+    int64_t cap = (int64_t)albd_num_cus(c) * 32;
     hipLaunchKernelGGL(k_to_vertex_order, dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))),
                        dim3(kBlock), 0, albd_stream(c), *g, *st);
     HIPCHK(hipGetLastError());
