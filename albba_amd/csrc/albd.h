@@ -56,6 +56,8 @@ typedef struct albd_counters {
     int persist_in_queue;              /* 1: its last frontier is in st.queue   */
     int persist_error;                 /* 1: its grid barrier gave up           */
     int pad_;
+    unsigned int seq;                  /* host mailbox: sequence of the copy    */
+    unsigned int pad2_;
 } albd_counters;
 
 /* levels one small-frontier call may run (st.level_log holds 4 per level) */
@@ -205,6 +207,15 @@ ALBD_HIDDEN int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_sta
  * (MLP argmin when sel->params is set, else the reference heuristic) */
 ALBD_HIDDEN int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st,
                             const albd_select_args *sel);
+/* albd_select, then the counters on the host (albd_host_counters): the
+ * kernel posts them into a pinned host-coherent mailbox and the host polls
+ * it, instead of a copy and a stream synchronisation (synchronous) */
+ALBD_HIDDEN int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                                 const albd_select_args *sel);
+/* mailbox plumbing for albd_select_read: the next sequence number and the
+ * mailbox's device address; the wait fills albd_host_counters */
+ALBD_HIDDEN albd_counters *albd_mail_next(albd_ctx *c, unsigned int *seq);
+ALBD_HIDDEN int albd_mail_wait(albd_ctx *c, const albd_state *st, unsigned int seq);
 
 /* ---- batched BFS (batch.hip) ---- */
 /* One instance: a device graph, its own scratch, a root.  The control block

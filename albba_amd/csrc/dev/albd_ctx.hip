@@ -31,6 +31,9 @@ struct albd_ctx {
     hipStream_t stream;
     hipEvent_t ev[ALBD_NEVENTS];
     albd_counters *hctr; // pinned
+    albd_counters *mail; // pinned, host-coherent, written by the select kernel
+    albd_counters *dmail; // its device address
+    unsigned int mail_seq;
 };
 
 int albd_device_count(int *count)
@@ -62,6 +65,10 @@ int albd_ctx_create(int device, albd_ctx **out)
     for (int i = 0; i < ALBD_NEVENTS; ++i) HIPCHK(hipEventCreate(&c->ev[i]));
     HIPCHK(hipHostMalloc((void **)&c->hctr, sizeof(albd_counters), hipHostMallocDefault));
     memset(c->hctr, 0, sizeof(albd_counters));
+    HIPCHK(hipHostMalloc((void **)&c->mail, sizeof(albd_counters),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    memset(c->mail, 0, sizeof(albd_counters));
+    HIPCHK(hipHostGetDevicePointer((void **)&c->dmail, c->mail, 0));
     *out = c;
     return 0;
 }
@@ -74,6 +81,7 @@ void albd_ctx_destroy(albd_ctx *c)
     for (int i = 0; i < ALBD_NEVENTS; ++i) (void)hipEventDestroy(c->ev[i]);
     (void)hipStreamDestroy(c->stream);
     (void)hipHostFree(c->hctr);
+    (void)hipHostFree(c->mail);
     delete c;
 }
 
@@ -142,6 +150,33 @@ int albd_read_counters(albd_ctx *c, const albd_state *st)
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+albd_counters *albd_mail_next(albd_ctx *c, unsigned int *seq)
+{
+    *seq = ++c->mail_seq;
+    if (*seq == 0) *seq = ++c->mail_seq; // 0 is the mailbox's initial value
+    return c->dmail;
+}
+
+int albd_mail_wait(albd_ctx *c, const albd_state *st, unsigned int seq)
+{
+    volatile unsigned int *sp = &c->mail->seq;
+    for (unsigned long spins = 0;; ++spins) {
+        if (__atomic_load_n(sp, __ATOMIC_ACQUIRE) == seq) {
+            memcpy(c->hctr, (const void *)c->mail, sizeof(albd_counters));
+            return 0;
+        }
+        if ((spins & 1023) == 1023) {
+            // the stream failed, or finished without the post becoming
+            // visible: take the counters the ordinary way
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return albd_set_error(e, "select kernel", __FILE__, __LINE__);
+            if (__atomic_load_n(sp, __ATOMIC_ACQUIRE) == seq) continue;
+            return albd_read_counters(c, st);
+        }
+    }
 }
 
 int albd_event_record(albd_ctx *c, int idx)

@@ -336,7 +336,8 @@ __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 // level.  One block.
 constexpr int kSelectBlock = 256;
 
-__global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel)
+__global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel,
+                                                         albd_counters *mail, unsigned int seq)
 {
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
     __shared__ float s_params[ALBBA_MLP_NPARAM];
@@ -400,6 +401,15 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         st.ctr->heur_choice = heur;
         st.ctr->sel_choice = choice >= 0 ? choice : heur;
         for (int i = 0; i < ALBBA_MLP_IN; ++i) st.ctr->features[i] = xs[i];
+        if (mail) {
+            // the whole block to host memory, then the sequence number with a
+            // system-scope release: the host sees the data once it sees seq
+            const unsigned long long *src = reinterpret_cast<const unsigned long long *>(st.ctr);
+            unsigned long long *dst = reinterpret_cast<unsigned long long *>(mail);
+            const int words = (int)(offsetof(albd_counters, seq) / 8);
+            for (int i = 0; i < words; ++i) dst[i] = src[i];
+            __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -1190,9 +1200,21 @@ int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int3
 
 int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st, const albd_select_args *sel)
 {
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel,
+                       (albd_counters *)nullptr, 0u);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                     const albd_select_args *sel)
+{
+    unsigned int seq;
+    albd_counters *mail = albd_mail_next(c, &seq);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel, mail,
+                       seq);
+    HIPCHK(hipGetLastError());
+    return albd_mail_wait(c, st, seq);
 }
 
 int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level)

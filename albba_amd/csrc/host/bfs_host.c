@@ -492,7 +492,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     int rc;
     /* init, then the select kernel turns the root into level 0's frontier */
     if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)) ||
-        (rc = albd_select(d, &G->g, &G->st, &sel)) || (rc = albd_read_counters(d, &G->st)))
+        (rc = albd_select_read(d, &G->g, &G->st, &sel)))
         return albba_fail(rc, "bfs: %s", albd_error());
     const albd_counters *hc = albd_host_counters(d);
     int64_t frontier = (int64_t)hc->frontier;
@@ -561,7 +561,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         sel.level = level + nlev;
         sel.prev_choice = choice;
         sel.first = 0;
-        if ((rc = albd_select(d, &G->g, &G->st, &sel)) || (rc = albd_read_counters(d, &G->st)))
+        if ((rc = albd_select_read(d, &G->g, &G->st, &sel)))
             return albba_fail(rc, "bfs: %s", albd_error());
         level_sums sums;
         sum_counters(hc, &sums);
