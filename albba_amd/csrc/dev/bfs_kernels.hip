@@ -57,8 +57,15 @@ __device__ __forceinline__ void slot_add(albd_slot *slots, unsigned long long cn
 // ---------------------------------------------------------------------------
 // pull
 
+// The column-bypass pull stages a frontier summary of up to 32 KB in LDS per
+// block; its blocks are 1024 threads so that two of them (32 waves) share a
+// CU, where 256-thread blocks would be held to 16 waves by the LDS.
+constexpr int kCBBlock = 1024;
+
+template <int VAR> constexpr int pull_block() { return VAR == ALBD_PULL_LIST_CB ? kCBBlock : kBlock; }
+
 template <int C, int VAR>
-__global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
+__global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
 {
     extern __shared__ uint32_t lds_sum[];
     const albd_graph &g = a.g;
@@ -66,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullArgs a)
 
     if constexpr (VAR == ALBD_PULL_LIST_CB) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(a.st.summary);
-        for (int i = threadIdx.x; i < a.sum_words32; i += kBlock) lds_sum[i] = src[i];
+        for (int i = threadIdx.x; i < a.sum_words32; i += pull_block<VAR>()) lds_sum[i] = src[i];
         __syncthreads();
     }
     const int64_t nitems = (VAR == ALBD_PULL_ALL) ? (int64_t)g.nwords : (int64_t)(*a.nitems);
@@ -1006,7 +1013,7 @@ int launch_pull_c(albd_ctx *c, PullArgs a, int variant, size_t lds, int64_t item
     if (variant == ALBD_PULL_LIST_CB) {
         // the LDS summary is staged once per block: fewer, longer blocks
         int per_cu = (int)(160 * 1024 / (lds + 1024));
-        per_cu = per_cu > 8 ? 8 : (per_cu < 1 ? 1 : per_cu);
+        per_cu = per_cu > 2 ? 2 : (per_cu < 1 ? 1 : per_cu);
         const int64_t resident = (int64_t)albd_num_cus(c) * per_cu;
         a.per_block = (int32_t)((items + resident - 1) / resident);
         if (a.per_block < 16) a.per_block = 16;
@@ -1021,7 +1028,7 @@ int launch_pull_c(albd_ctx *c, PullArgs a, int variant, size_t lds, int64_t item
     else if (variant == ALBD_PULL_LIST)
         hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST_CB>), dim3((unsigned)blocks), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((k_pull<C, ALBD_PULL_LIST_CB>), dim3((unsigned)blocks), dim3(kCBBlock), lds, s, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
