@@ -420,7 +420,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
     }
 }
 
-enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4 };
+// LB_SMALL: the small-frontier kernel follows; zero its grid state, level 0's
+// counters and the error flag here instead of in launches of their own
+enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4, LB_SMALL = 8 };
 
 __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t nwords, int flags)
 {
@@ -438,6 +440,13 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
             if (flags & LB_ACTIVE_FROM_NEXT) st.ctr->active_len = st.ctr->active_next;
             st.ctr->qnext_len = 0;
             st.ctr->active_next = 0;
+            if (flags & LB_SMALL) {
+                st.persist->found_total = 0;
+                st.persist->barrier = 0;
+                st.persist->error = 0;
+                for (int f = 0; f < 4; ++f) st.level_log[f] = 0;
+                st.ctr->persist_error = 0;
+            }
         }
     }
 }
@@ -1183,17 +1192,16 @@ int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int3
     if (max_levels < 1) max_levels = 1;
     if (albd_coop_off(c)) return ALBD_NOCOOP;
     // every workgroup of the grid barrier must be resident at once
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
+    static int per_cu = -1; // the same for every device of this build
+    if (per_cu < 0)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
     const int cap = per_cu * albd_num_cus(c);
     if (blocks <= 0) blocks = 64;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) return albd_set_error_msg(ALBBA_EHIP, "small-frontier kernel cannot be resident");
     hipStream_t s = albd_stream(c);
-    // level 0's counters, the grid state and the error flag start at zero
-    if (int rc = albd_fill_u64(c, st->level_log, 0, 4)) return rc;
-    if (int rc = albd_fill_u64(c, st->persist, 0, sizeof(albd_persist) / 8)) return rc;
-    if (int rc = albd_fill_u32(c, &st->ctr->persist_error, 0, 1)) return rc;
+    // level 0's counters, the grid state and the error flag were zeroed by
+    // the level-begin kernel (LB_SMALL)
     SmallArgs a{*g, *st, level0, max_levels, edge_cap, vertex_cap, spin_limit};
     void *args[] = {&a};
     if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_push_small), dim3(blocks),

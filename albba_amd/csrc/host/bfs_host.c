@@ -286,7 +286,7 @@ void albba_bfs_opts_default(albba_bfs_opts *o)
     o->small_frontier_edges = 65536;
 }
 
-enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4 };
+enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4, LB_SMALL = 8 };
 int albd_level_begin_flags(albd_ctx *c, const albd_graph *g, const albd_state *st, int flags);
 
 typedef struct loop_state {
@@ -363,6 +363,7 @@ static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_
     if (ls->prev_push) flags |= LB_QUEUE_FROM_NEXT;
     if (ls->list_pending) flags |= LB_ACTIVE_FROM_NEXT;
     if (!ls->have_queue) flags |= LB_ZERO_QUEUE;
+    flags |= LB_SMALL;
     int rc = albd_level_begin_flags(d, &G->g, &G->st, flags);
     if (rc) return rc;
     if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
