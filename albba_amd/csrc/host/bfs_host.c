@@ -489,6 +489,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     /* the heuristic pushes while F <= (int)(n * 0.04) */
     const int64_t push_vertex_cap = (int64_t)(int)((double)G->n * 0.04);
     loop_state ls = {1, 0, 0, 0, 0};
+    int small_streak = 0; /* consecutive plain push levels that stayed small */
     int32_t level = 0;
     int rc;
     /* init, then the select kernel turns the root into level 0's frontier */
@@ -538,15 +539,20 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                 (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))))
                 return albba_fail(rc, "bfs: %s", albd_error());
         }
-        /* a pushed level with a small frontier starts a run of push levels in
-         * the single-workgroup kernel; it continues while the frontier stays
-         * within the edge cap and within the heuristic's push threshold
-         * (bfs/src/bfs_SELLCSv2.c:282-284), so the heuristic's choices are
-         * unchanged */
+        /* A pushed level with a small frontier, after two more such levels,
+         * starts a run of push levels in the cooperative kernel; it continues
+         * while the frontier stays within the edge cap and within the
+         * heuristic's push threshold (bfs/src/bfs_SELLCSv2.c:282-284), so the
+         * heuristic's choices are unchanged.  The cooperative launch costs
+         * more than one level: it pays off on slowly growing frontiers
+         * (high-diameter graphs), which the streak detects, not on the one or
+         * two small levels of a power-law BFS. */
         const int64_t first_edges = (int64_t)hc->frontier_edges;
-        const int small = choice == ALBBA_CHOICE_PUSH && opts.small_frontier_edges > 0 &&
-                          (policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED) &&
-                          first_edges <= opts.small_frontier_edges && frontier <= push_vertex_cap;
+        const int small_ok = choice == ALBBA_CHOICE_PUSH && opts.small_frontier_edges > 0 &&
+                             (policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED) &&
+                             first_edges <= opts.small_frontier_edges && frontier <= push_vertex_cap;
+        const int small = small_ok && small_streak >= 2;
+        small_streak = small_ok && !small ? small_streak + 1 : 0;
         int nlev = 1;
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
         if (small)

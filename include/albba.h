@@ -214,11 +214,11 @@ typedef struct albba_bfs_opts {
     int32_t time_levels;         /* 1: record per-level kernel times (events)      */
     int32_t timing_reps;         /* ALBBA_SWITCH_BEST / sampling: reps per choice  */
     /* Heuristic and learned policies: when a level pushes and its frontier has
-     * at most this many out-edges, consecutive levels run inside one
-     * cooperative push kernel (one grid barrier per level, no host round
-     * trip) until the frontier outgrows the cap or the heuristic's push
-     * threshold (0 disables; default 65536).  Results are identical either
-     * way. */
+     * at most this many out-edges, after two such levels in a row (a slowly
+     * growing frontier), the following levels run inside one cooperative push
+     * kernel (one grid barrier per level, no host round trip) until the
+     * frontier outgrows the cap or the heuristic's push threshold (0
+     * disables; default 65536).  Results are identical either way. */
     int64_t small_frontier_edges;
     int32_t small_frontier_blocks; /* workgroups of that kernel (0: default)    */
     int32_t reserved_;
