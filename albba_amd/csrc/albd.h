@@ -161,7 +161,7 @@ ALBD_HIDDEN int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state
                               int32_t rp, int32_t ro);
 /* level/parent from position order (st->lp) to vertex order */
 ALBD_HIDDEN int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st);
-/* zero st->next and the per-level slot counters */
+/* zero st->next (and, flag 1, the per-level slot counters) */
 ALBD_HIDDEN int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_state *st);
 
 /* pull variants */
@@ -171,6 +171,10 @@ enum { ALBD_PULL_ALL = 0, ALBD_PULL_LIST = 1, ALBD_PULL_LIST_CB = 2 };
 ALBD_HIDDEN int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st,
                                   int from_list, int64_t prev_len);
 ALBD_HIDDEN int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* one launch: zero st->next, compact the unvisited-word list (as
+ * albd_build_active), and with `summary` build the frontier summary */
+ALBD_HIDDEN int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list,
+                          int summary);
 ALBD_HIDDEN int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant,
                           int32_t level, int64_t items_bound);
 ALBD_HIDDEN int albd_cb_supported(const albd_graph *g);
@@ -212,10 +216,15 @@ ALBD_HIDDEN int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *
  * it, instead of a copy and a stream synchronisation (synchronous) */
 ALBD_HIDDEN int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                  const albd_select_args *sel);
+/* the launch half of albd_select_read (asynchronous); albd_mail_wait(seq)
+ * completes it */
+ALBD_HIDDEN int albd_select_post(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                                 const albd_select_args *sel, unsigned int *seq);
 /* mailbox plumbing for albd_select_read: the next sequence number and the
  * mailbox's device address; the wait fills albd_host_counters */
 ALBD_HIDDEN albd_counters *albd_mail_next(albd_ctx *c, unsigned int *seq);
 ALBD_HIDDEN int albd_mail_wait(albd_ctx *c, const albd_state *st, unsigned int seq);
+ALBD_HIDDEN void albd_mail_stats(const albd_ctx *c, unsigned long *hits, unsigned long *misses);
 
 /* ---- batched BFS (batch.hip) ---- */
 /* One instance: a device graph, its own scratch, a root.  The control block

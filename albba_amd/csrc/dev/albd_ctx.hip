@@ -34,6 +34,7 @@ struct albd_ctx {
     albd_counters *mail; // pinned, host-coherent, written by the select kernel
     albd_counters *dmail; // its device address
     unsigned int mail_seq;
+    unsigned long mail_hits, mail_misses; // posts seen / waits that fell back
 };
 
 int albd_device_count(int *count)
@@ -165,6 +166,7 @@ int albd_mail_wait(albd_ctx *c, const albd_state *st, unsigned int seq)
     for (unsigned long spins = 0;; ++spins) {
         if (__atomic_load_n(sp, __ATOMIC_ACQUIRE) == seq) {
             memcpy(c->hctr, (const void *)c->mail, sizeof(albd_counters));
+            ++c->mail_hits;
             return 0;
         }
         if ((spins & 1023) == 1023) {
@@ -174,9 +176,16 @@ int albd_mail_wait(albd_ctx *c, const albd_state *st, unsigned int seq)
             if (e == hipErrorNotReady) continue;
             if (e != hipSuccess) return albd_set_error(e, "select kernel", __FILE__, __LINE__);
             if (__atomic_load_n(sp, __ATOMIC_ACQUIRE) == seq) continue;
+            ++c->mail_misses;
             return albd_read_counters(c, st);
         }
     }
+}
+
+void albd_mail_stats(const albd_ctx *c, unsigned long *hits, unsigned long *misses)
+{
+    *hits = c->mail_hits;
+    *misses = c->mail_misses;
 }
 
 int albd_event_record(albd_ctx *c, int idx)

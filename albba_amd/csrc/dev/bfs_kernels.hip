@@ -356,13 +356,16 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         for (int i = tid; i < ALBBA_MLP_NPARAM; i += kSelectBlock) s_params[i] = sel.params[i];
     unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
     for (int i = tid; i < ALBD_NSLOT; i += kSelectBlock) {
-        const albd_slot &sl = st.slots[i];
+        albd_slot &sl = st.slots[i];
         F += sl.next_count;
         mf += sl.next_edges;
         nzw += sl.next_words;
         scn += sl.scanned;
         wds += sl.words;
         byp += sl.bypassed;
+        // read once, by this thread only: the next level starts from zero
+        sl.next_count = sl.next_edges = sl.next_words = 0;
+        sl.scanned = sl.words = sl.bypassed = 0;
     }
     F = wave_sum(F);
     mf = wave_sum(mf);
@@ -417,29 +420,37 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
             for (int i = 0; i < words; ++i) dst[i] = src[i];
             __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
+        // hand the level's lists over to the next level: a pushed level's
+        // queue becomes the queue (after a pull the queue is rebuilt from
+        // the bitmap, from zero); a used unvisited-word list becomes the list
+        if (!sel.first)
+            st.ctr->queue_len = sel.prev_choice == ALBBA_CHOICE_PUSH ? st.ctr->qnext_len : 0ull;
+        st.ctr->qnext_len = 0;
+        if (sel.prev_choice == ALBBA_CHOICE_PULL_SB || sel.prev_choice == ALBBA_CHOICE_PULL_CB)
+            st.ctr->active_len = st.ctr->active_next;
+        st.ctr->active_next = 0;
     }
 }
 
-// LB_SMALL: the small-frontier kernel follows; zero its grid state, level 0's
-// counters and the error flag here instead of in launches of their own
-enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4, LB_SMALL = 8 };
+// Zero st.next (the select kernel has moved the counters and zeroed the
+// slots).  LB_SLOTS: zero the slots too (trial runs of one level);
+// LB_SMALL: the small-frontier kernel follows; zero its grid state, level
+// 0's counters and the error flag here instead of in launches of their own.
+enum { LB_SLOTS = 1, LB_SMALL = 8 };
 
 __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t nwords, int flags)
 {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords;
          i += (int64_t)gridDim.x * kBlock)
         st.next[i] = 0;
-    unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-         i < (int64_t)(ALBD_NSLOT * sizeof(albd_slot) / 8); i += (int64_t)gridDim.x * kBlock)
-        slots[i] = 0;
+    if (flags & LB_SLOTS) {
+        unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+             i < (int64_t)(ALBD_NSLOT * sizeof(albd_slot) / 8); i += (int64_t)gridDim.x * kBlock)
+            slots[i] = 0;
+    }
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
-            if (flags & LB_QUEUE_FROM_NEXT) st.ctr->queue_len = st.ctr->qnext_len;
-            if (flags & LB_ZERO_QUEUE) st.ctr->queue_len = 0;
-            if (flags & LB_ACTIVE_FROM_NEXT) st.ctr->active_len = st.ctr->active_next;
-            st.ctr->qnext_len = 0;
-            st.ctr->active_next = 0;
             if (flags & LB_SMALL) {
                 st.persist->found_total = 0;
                 st.persist->barrier = 0;
@@ -454,10 +465,9 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
 // unvisited-word list: from every word (FROM_LIST = false) or from the
 // previous list (visited only grows, so an old list is a superset)
 template <bool FROM_LIST>
-__global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
+__device__ __forceinline__ void active_body(const albd_graph &g, const albd_state &st,
+                                            unsigned int *s_cnt, unsigned long long *s_base)
 {
-    __shared__ unsigned int s_cnt[kWavesPerBlock];
-    __shared__ unsigned long long s_base;
     constexpr int kGroups = 4; // 64-item groups per wave per step
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t n_in = FROM_LIST ? (int64_t)st.ctr->active_len : (int64_t)g.nwords;
@@ -478,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
             m[k] = __ballot(keep);
             cnt += (unsigned int)__popcll(m[k]);
         }
-        unsigned long long off = block_reserve(&st.ctr->active_next, cnt, s_cnt, &s_base);
+        unsigned long long off = block_reserve(&st.ctr->active_next, cnt, s_cnt, s_base);
 #pragma unroll
         for (int k = 0; k < kGroups; ++k) {
             if ((m[k] >> lane) & 1ull) st.active_next[off + __popcll(m[k] & lanemask_lt(lane))] = w[k];
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_summary(albd_graph g, albd_state st)
+__device__ __forceinline__ void summary_body(const albd_graph &g, const albd_state &st)
 {
     const int lane = lane_id();
     const int64_t ngroups = ((int64_t)g.nwords + 63) >> 6;
@@ -499,6 +509,34 @@ __global__ __launch_bounds__(kBlock) void k_summary(albd_graph g, albd_state st)
         const uint64_t m = __ballot(nz);
         if (lane == 0) st.summary[grp] = m;
     }
+}
+
+template <bool FROM_LIST>
+__global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
+{
+    __shared__ unsigned int s_cnt[kWavesPerBlock];
+    __shared__ unsigned long long s_base;
+    active_body<FROM_LIST>(g, st, s_cnt, &s_base);
+}
+
+__global__ __launch_bounds__(kBlock) void k_summary(albd_graph g, albd_state st)
+{
+    summary_body(g, st);
+}
+
+// The next level's preparation in one launch: zero st.next, compact the
+// unvisited-word list, build the frontier summary.  The three parts touch
+// disjoint data; the counters they use were set by the select kernel.
+template <bool FROM_LIST, bool SUMMARY>
+__global__ __launch_bounds__(kBlock) void k_prep(albd_graph g, albd_state st)
+{
+    __shared__ unsigned int s_cnt[kWavesPerBlock];
+    __shared__ unsigned long long s_base;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < g.nwords;
+         i += (int64_t)gridDim.x * kBlock)
+        st.next[i] = 0;
+    active_body<FROM_LIST>(g, st, s_cnt, &s_base);
+    if constexpr (SUMMARY) summary_body(g, st);
 }
 
 // Words -> queue of positions with out-degree > 0.  FINALIZE (after a push):
@@ -1115,6 +1153,21 @@ int albd_cb_supported(const albd_graph *g)
     return bytes <= 64 * 1024;
 }
 
+int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list, int summary)
+{
+    const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
+    hipStream_t s = albd_stream(c);
+    if (from_list) {
+        if (summary) hipLaunchKernelGGL((k_prep<true, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        else hipLaunchKernelGGL((k_prep<true, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+    } else {
+        if (summary) hipLaunchKernelGGL((k_prep<false, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        else hipLaunchKernelGGL((k_prep<false, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
@@ -1221,14 +1274,21 @@ int albd_select(albd_ctx *c, const albd_graph *g, const albd_state *st, const al
     return 0;
 }
 
+int albd_select_post(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                     const albd_select_args *sel, unsigned int *seq)
+{
+    albd_counters *mail = albd_mail_next(c, seq);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel, mail,
+                       *seq);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
                      const albd_select_args *sel)
 {
     unsigned int seq;
-    albd_counters *mail = albd_mail_next(c, &seq);
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, albd_stream(c), *g, *st, *sel, mail,
-                       seq);
-    HIPCHK(hipGetLastError());
+    if (int rc = albd_select_post(c, g, st, sel, &seq)) return rc;
     return albd_mail_wait(c, st, seq);
 }
 

@@ -51,6 +51,11 @@ int albba_ctx_create(int device, albba_ctx **out)
 void albba_ctx_destroy(albba_ctx *c)
 {
     if (!c) return;
+    if (getenv("ALBBA_VERBOSE")) {
+        unsigned long hits = 0, misses = 0;
+        albd_mail_stats(c->d, &hits, &misses);
+        fprintf(stderr, "[albba] counter mailbox: %lu posts seen, %lu fallbacks\n", hits, misses);
+    }
     albd_ctx_destroy(c->d);
     free(c);
 }
@@ -286,7 +291,9 @@ void albba_bfs_opts_default(albba_bfs_opts *o)
     o->small_frontier_edges = 65536;
 }
 
-enum { LB_QUEUE_FROM_NEXT = 1, LB_ZERO_QUEUE = 2, LB_ACTIVE_FROM_NEXT = 4, LB_SMALL = 8 };
+/* k_level_begin flags; the select kernel moves the queue/list counters from
+ * one level to the next and zeroes the slots */
+enum { LB_SLOTS = 1, LB_SMALL = 8 };
 int albd_level_begin_flags(albd_ctx *c, const albd_graph *g, const albd_state *st, int flags);
 
 typedef struct loop_state {
@@ -323,11 +330,8 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
                      int64_t frontier, int mark)
 {
     albd_ctx *d = G->ctx->d;
-    int flags = 0;
-    if (ls->prev_push) flags |= LB_QUEUE_FROM_NEXT;
-    if (ls->list_pending) flags |= LB_ACTIVE_FROM_NEXT;
-    if (choice == ALBBA_CHOICE_PUSH && !ls->have_queue) flags |= LB_ZERO_QUEUE;
-    int rc = albd_level_begin_flags(d, &G->g, &G->st, flags);
+    /* slots too: trial runs of one level start from zero each time */
+    int rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SLOTS);
     if (rc) return rc;
     if (choice == ALBBA_CHOICE_PUSH) {
         if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
@@ -351,20 +355,51 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
     return mark ? albd_event_record(d, 7) : 0;
 }
 
+/* Choice-independent preparation of the next level, launched behind the
+ * select kernel so that it runs while the host waits for the choice: zero
+ * `next` and the slots, move the queue/list counters, compact the
+ * unvisited-word list (slice bypass) and build the frontier summary (column
+ * bypass).  A list that the chosen kernel does not use is dropped. */
+static int prep_level(albba_graph *G, const loop_state *ls)
+{
+    /* one launch sized for every word (the list length is on the device) */
+    return albd_prep(G->ctx->d, &G->g, &G->st, ls->list_valid, albd_cb_supported(&G->g));
+}
+
+/* The kernels of one level after prep_level (no pointer swaps). */
+static int run_main(albba_graph *G, const loop_state *ls, int choice, int32_t level,
+                    int64_t frontier, int mark)
+{
+    albd_ctx *d = G->ctx->d;
+    int rc;
+    if (choice == ALBBA_CHOICE_PUSH) {
+        if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
+        if ((rc = albd_push_scan(d, &G->g, &G->st, frontier))) return rc;
+        if (mark && (rc = albd_event_record(d, 6))) return rc;
+        if ((rc = albd_push_expand(d, &G->g, &G->st))) return rc;
+        if (mark && (rc = albd_event_record(d, 7))) return rc;
+        return albd_push_finalize(d, &G->g, &G->st, level);
+    }
+    int variant = ALBD_PULL_ALL;
+    if (choice == ALBBA_CHOICE_PULL_SB) variant = ALBD_PULL_LIST;
+    if (choice == ALBBA_CHOICE_PULL_CB) variant = ALBD_PULL_LIST_CB;
+    if (mark && (rc = albd_event_record(d, 6))) return rc;
+    const int64_t bound = ls->list_valid ? ls->list_len : G->g.nwords;
+    if ((rc = albd_pull(d, &G->g, &G->st, variant, level, bound))) return rc;
+    return mark ? albd_event_record(d, 7) : 0;
+}
+
 /* Push levels in the single-workgroup kernel, starting with the level whose
  * frontier is st.queue (or the front bitmap after a pull).  Returns the
  * number of levels run in *nlev; afterwards the state is that of one
  * finished push level (queue_next holds the last frontier's queue). */
 static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_t frontier,
-                     int64_t edge_cap, int64_t vertex_cap, int32_t blocks, int mark, int *nlev)
+                     int64_t edge_cap, int64_t vertex_cap, int32_t blocks, int mark, int prepped,
+                     int *nlev)
 {
     albd_ctx *d = G->ctx->d;
-    int flags = 0;
-    if (ls->prev_push) flags |= LB_QUEUE_FROM_NEXT;
-    if (ls->list_pending) flags |= LB_ACTIVE_FROM_NEXT;
-    if (!ls->have_queue) flags |= LB_ZERO_QUEUE;
-    flags |= LB_SMALL;
-    int rc = albd_level_begin_flags(d, &G->g, &G->st, flags);
+    (void)prepped; /* the counters moved in the select kernel either way */
+    int rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SMALL);
     if (rc) return rc;
     if (!ls->have_queue && (rc = albd_front_to_queue(d, &G->g, &G->st))) return rc;
     if (mark && (rc = albd_event_record(d, 6))) return rc;
@@ -400,8 +435,9 @@ static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_
     return 0;
 }
 
-/* Pointer swaps and bookkeeping after a committed level. */
-static void commit_level(albba_graph *G, loop_state *ls, int choice, const level_sums *sums)
+/* Pointer swaps and bookkeeping after a committed level (list length from
+ * the level's counters: commit_counts). */
+static void commit_pointers(albba_graph *G, loop_state *ls, int choice)
 {
     uint64_t *t = G->st.front;
     G->st.front = G->st.next;
@@ -425,9 +461,13 @@ static void commit_level(albba_graph *G, loop_state *ls, int choice, const level
             G->st.active_next = a;
             ls->list_valid = 1;
             ls->list_pending = 1;
-            ls->list_len = sums->active_next;
         }
     }
+}
+
+static void commit_counts(loop_state *ls, int choice, const level_sums *sums)
+{
+    if (choice == ALBBA_CHOICE_PULL_SB || choice == ALBBA_CHOICE_PULL_CB) ls->list_len = sums->active_next;
 }
 
 static int choice_ok(const albba_graph *G, int c)
@@ -492,10 +532,20 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     int small_streak = 0; /* consecutive plain push levels that stayed small */
     int32_t level = 0;
     int rc;
+    /* The heuristic and learned policies prepare every level speculatively
+     * behind the select kernel (prep_level); the others prepare each level
+     * after their choice (run_level). */
+    const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED;
+    unsigned int seq = 0;
     /* init, then the select kernel turns the root into level 0's frontier */
-    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)) ||
-        (rc = albd_select_read(d, &G->g, &G->st, &sel)))
+    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
+    if (spec)
+        rc = albd_select_post(d, &G->g, &G->st, &sel, &seq) || prep_level(G, &ls) ||
+             albd_mail_wait(d, &G->st, seq);
+    else
+        rc = albd_select_read(d, &G->g, &G->st, &sel);
+    if (rc) return albba_fail(ALBBA_EHIP, "bfs: %s", albd_error());
     const albd_counters *hc = albd_host_counters(d);
     int64_t frontier = (int64_t)hc->frontier;
 
@@ -557,7 +607,9 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
         if (small)
             rc = run_small(G, &ls, level, frontier, opts.small_frontier_edges, push_vertex_cap,
-                           opts.small_frontier_blocks, opts.time_levels, &nlev);
+                           opts.small_frontier_blocks, opts.time_levels, spec, &nlev);
+        else if (spec)
+            rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
         else
             rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
         if (rc) return albba_fail(rc, "bfs: %s", albd_error());
@@ -568,8 +620,17 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         sel.level = level + nlev;
         sel.prev_choice = choice;
         sel.first = 0;
-        if ((rc = albd_select_read(d, &G->g, &G->st, &sel)))
+        if (spec) {
+            /* swap to the next level's state, post the select kernel, queue
+             * the next level's preparation behind it, then wait for the
+             * counters */
+            commit_pointers(G, &ls, choice);
+            if ((rc = albd_select_post(d, &G->g, &G->st, &sel, &seq)) || (rc = prep_level(G, &ls)) ||
+                (rc = albd_mail_wait(d, &G->st, seq)))
+                return albba_fail(rc, "bfs: %s", albd_error());
+        } else if ((rc = albd_select_read(d, &G->g, &G->st, &sel))) {
             return albba_fail(rc, "bfs: %s", albd_error());
+        }
         level_sums sums;
         sum_counters(hc, &sums);
         if (stats && level < stats->capacity) {
@@ -604,7 +665,8 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                 if (stats->bypassed) stats->bypassed[lv] = push ? 0 : sums.bypassed;
             }
         }
-        commit_level(G, &ls, choice, &sums);
+        if (!spec) commit_pointers(G, &ls, choice);
+        commit_counts(&ls, choice, &sums);
         frontier = (int64_t)hc->frontier;
         level += nlev;
     }
