@@ -16,8 +16,9 @@
 // Push (the SpMSpV slot of bfs/src/bfs_par_kernels.c:566-656, done here as a
 // true frontier-driven expansion): edges of the frontier queue are split into
 // 64-edge chunks via a degree prefix sum; each edge into a vertex unvisited
-// at the start of the level does atomicMin(pmin[v], id(u)) and sets v in the
-// next bitmap; a word pass then writes level/parent and builds the next queue.
+// at the start of the level does atomicMin(pmin[v], id(u)); the edge that
+// finds pmin[v] still all ones sets v in the next bitmap; a word pass then
+// writes level/parent and builds the next queue.
 #include "dev_common.h"
 #include "select_dev.h"
 #include "../../../include/albba.h"
@@ -789,9 +790,11 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
             const uint32_t ou = (uint32_t)g.row_order[u];
             const int32_t v = g.push_col[g.push_ptr[u] + (e - start)];
             if (!((st.visited[v >> 6] >> (v & 63)) & 1ull)) {
-                atomicMin(&st.pmin[v], ou);
-                atomicOr(reinterpret_cast<unsigned long long *>(&st.next[v >> 6]),
-                         1ull << (v & 63));
+                // pmin is all ones at rest: the first edge into v this level
+                // sees that value back and is the one that sets v's bit
+                if (atomicMin(&st.pmin[v], ou) == 0xFFFFFFFFu)
+                    atomicOr(reinterpret_cast<unsigned long long *>(&st.next[v >> 6]),
+                             1ull << (v & 63));
             }
         }
         // the next chunk starts at edge e0 + 64; its vertex is v0 plus the
