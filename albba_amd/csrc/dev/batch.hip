@@ -133,7 +133,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
     if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
 
     // team-uniform loop state
-    unsigned long long F = 1, mf = (unsigned long long)rdeg, nzw = 1;
+    unsigned long long F = 1, mf = (unsigned long long)rdeg, nzw = 1, F_prev = 0;
     unsigned long long reached = 1, redges = (unsigned long long)rdeg;
     int prev_choice = ALBBA_CHOICE_PUSH;
     int64_t fs_cur = 0, fe_cur = 1, fs_prev = 0, fe_prev = 0; // found ranges of B_L, B_{L-1}
@@ -155,7 +155,8 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
             (unsigned long long)g.nnz > redges ? (unsigned long long)g.nnz - redges : 0ull;
         if (learned && ((long long)F > thr || 16 * mf >= unexplored)) {
             if (tid == 0)
-                level_features(S.xs, g.n, g.nnz, nwords, F, mf, nzw, reached, redges, L, prev_choice);
+                level_features(S.xs, g.n, g.nnz, nwords, F, mf, nzw, reached, redges, L, prev_choice,
+                               F_prev);
             __syncthreads();
             const int c = mlp_argmin(S.params, S.xs, S.h1, S.h2, S.ys, 0xF, &S.choice);
             if (c >= 0) choice = c;
@@ -358,6 +359,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         fs_prev = fs_cur;
         fe_prev = fe_cur;
         prev_by_pull = cur_by_pull;
+        F_prev = F;
         if (push) {
             const int64_t total = (int64_t)ld_coherent(&it.ctl[1]);
             F = (unsigned long long)(total - fe_cur);

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
     const unsigned long long redges = st.ctr->reached_edges + mf;
     if (tid == 0)
         level_features(xs, g.n, g.nnz, g.nwords, F, mf, nzw, reached, redges, sel.level,
-                       sel.prev_choice);
+                       sel.prev_choice, st.ctr->frontier);
     __syncthreads();
     const int choice = sel.params ? mlp_argmin(s_params, xs, h1, h2, ys, sel.valid_mask, &s_choice) : -1;
     if (tid == 0) {

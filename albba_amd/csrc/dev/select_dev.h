@@ -9,11 +9,14 @@
 //   0 log2(1+F)/log2(1+n)   1 F/n   2 log2(1+mf)/log2(1+m)   3 mf/m
 //   4 unvisited vertices/n  5 unvisited edges/m  6 nonzero frontier words/words
 //   7 min(level,64)/64      8 log2(1+mf/F)/16    9 log2(1+m/n)/8
-//   10 log2(n)/32           11 previous level pushed (1) or pulled (0)
+//   10 log2(1+F/F_prev)/16 (frontier growth over the last level; scale-free,
+//      where log n would make graphs larger than the training set's an
+//      extrapolation)             11 previous level pushed (1) or pulled (0)
 __device__ inline void level_features(float *xs, int64_t n_, int64_t nnz, int64_t nwords,
                                       unsigned long long F, unsigned long long mf,
                                       unsigned long long nzw, unsigned long long reached,
-                                      unsigned long long redges, int32_t level, int32_t prev_choice)
+                                      unsigned long long redges, int32_t level, int32_t prev_choice,
+                                      unsigned long long F_prev)
 {
     const double n = (double)n_, m = nnz > 0 ? (double)nnz : 1.0;
     xs[0] = (float)(log2(1.0 + (double)F) / log2(1.0 + n));
@@ -26,7 +29,7 @@ __device__ inline void level_features(float *xs, int64_t n_, int64_t nnz, int64_
     xs[7] = (float)((level < 64 ? level : 64) / 64.0);
     xs[8] = (float)(log2(1.0 + (double)mf / (double)(F > 0 ? F : 1)) / 16.0);
     xs[9] = (float)(log2(1.0 + m / n) / 8.0);
-    xs[10] = (float)(log2(n) / 32.0);
+    xs[10] = (float)(log2(1.0 + (double)F / (double)(F_prev > 0 ? F_prev : 1)) / 16.0);
     xs[11] = prev_choice == ALBBA_CHOICE_PUSH ? 1.0f : 0.0f;
 }
 

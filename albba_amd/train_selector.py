@@ -92,8 +92,8 @@ def main():
     ap.add_argument("--data-out", default=None, help="save the dataset (.npz)")
     ap.add_argument("--data-in", default=None, help="train from a saved dataset")
     ap.add_argument("--scales", default="18,19,20,21,22")
-    ap.add_argument("--per-scale", type=int, default=3)
-    ap.add_argument("--roots", type=int, default=6)
+    ap.add_argument("--per-scale", type=int, default=5)
+    ap.add_argument("--roots", type=int, default=10)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--train-seed", type=int, default=8)
     ap.add_argument("--steps", type=int, default=4000)
