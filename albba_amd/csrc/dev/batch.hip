@@ -154,7 +154,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         const unsigned long long unexplored =
             (unsigned long long)g.nnz > redges ? (unsigned long long)g.nnz - redges : 0ull;
         if (learned && ((long long)F > thr || 16 * mf >= unexplored)) {
-            if (tid == 0)
+            if (tid < 64)
                 level_features(S.xs, g.n, g.nnz, nwords, F, mf, nzw, reached, redges, L, prev_choice,
                                F_prev);
             __syncthreads();
@@ -400,8 +400,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
 __global__ __launch_bounds__(kBB, 4) void k_bfs_batch(albd_batch_args a)
 {
     __shared__ BLds S;
-    if (a.params)
-        for (int i = threadIdx.x; i < ALBBA_MLP_NPARAM; i += kBB) S.params[i] = a.params[i];
+    if (a.params) mlp_to_lds<kBB>(S.params, a.params);
     __syncthreads();
     const int t = a.wg_team[blockIdx.x];
     const int rank = (int)blockIdx.x - a.team_first[t], T = a.team_size[t];

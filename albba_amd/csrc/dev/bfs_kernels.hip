@@ -347,26 +347,54 @@ constexpr int kSelectBlock = 256;
 __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel,
                                                          albd_counters *mail, unsigned int seq)
 {
+    // the counter block up to the mailbox sequence word, as 64-bit words
+    constexpr int kCtrWords = (int)(offsetof(albd_counters, seq) / 8);
+    static_assert(kCtrWords <= 64, "one wave moves the counter block");
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
     __shared__ float s_params[ALBBA_MLP_NPARAM];
     __shared__ unsigned long long s_red[6][kSelectBlock / 64];
-    __shared__ int s_choice;
+    __shared__ union {
+        albd_counters c;
+        unsigned long long w[sizeof(albd_counters) / 8];
+    } s_ctr;
     const int lane = lane_id(), wid = threadIdx.x >> 6, tid = threadIdx.x;
-    // the weights travel to LDS while the slots are summed
-    if (sel.params)
-        for (int i = tid; i < ALBBA_MLP_NPARAM; i += kSelectBlock) s_params[i] = sel.params[i];
+    unsigned long long *ctr_words = reinterpret_cast<unsigned long long *>(st.ctr);
+    // Every load first (the counter block, the weights, the slots), so that
+    // they travel together; then the block sums the slots, and wave 0 alone
+    // computes the features, runs the MLP and writes the counter block back
+    // (22 lanes in parallel: thread 0 alone would chain one load per word).
+    constexpr int kPer = ALBD_NSLOT / kSelectBlock;
+    static_assert(ALBD_NSLOT % kSelectBlock == 0, "slots split evenly over the block");
+    static_assert(sizeof(albd_slot) == 64, "slot = 3 x 16 counter bytes + padding");
+    unsigned long long cw = 0;
+    if (tid < kCtrWords) cw = ctr_words[tid];
+    float pr[mlp_per_thread<kSelectBlock>()];
+    if (sel.params) mlp_load<kSelectBlock>(pr, sel.params);
+    ulonglong2 v[kPer][3];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(&st.slots[tid + k * kSelectBlock]);
+        v[k][0] = p[0];
+        v[k][1] = p[1];
+        v[k][2] = p[2];
+    }
+    // read once, by this thread only: the next level starts from zero
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        ulonglong2 *p = reinterpret_cast<ulonglong2 *>(&st.slots[tid + k * kSelectBlock]);
+        p[0] = p[1] = p[2] = make_ulonglong2(0ull, 0ull);
+    }
+    if (tid < kCtrWords) s_ctr.w[tid] = cw;
+    if (sel.params) mlp_store<kSelectBlock>(s_params, pr);
     unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
-    for (int i = tid; i < ALBD_NSLOT; i += kSelectBlock) {
-        albd_slot &sl = st.slots[i];
-        F += sl.next_count;
-        mf += sl.next_edges;
-        nzw += sl.next_words;
-        scn += sl.scanned;
-        wds += sl.words;
-        byp += sl.bypassed;
-        // read once, by this thread only: the next level starts from zero
-        sl.next_count = sl.next_edges = sl.next_words = 0;
-        sl.scanned = sl.words = sl.bypassed = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        F += v[k][0].x;   // next_count
+        mf += v[k][0].y;  // next_edges
+        nzw += v[k][1].x; // next_words
+        scn += v[k][1].y; // scanned
+        wds += v[k][2].x; // words
+        byp += v[k][2].y; // bypassed
     }
     F = wave_sum(F);
     mf = wave_sum(mf);
@@ -383,6 +411,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         s_red[5][wid] = byp;
     }
     __syncthreads();
+    if (wid != 0) return;
+    albd_counters &c = s_ctr.c;
+    // the totals in every lane of wave 0
     F = mf = nzw = scn = wds = byp = 0;
     for (int i = 0; i < kSelectBlock / 64; ++i) {
         F += s_red[0][i];
@@ -392,44 +423,49 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         wds += s_red[4][i];
         byp += s_red[5][i];
     }
-    const unsigned long long reached = st.ctr->reached + F;
-    const unsigned long long redges = st.ctr->reached_edges + mf;
-    if (tid == 0)
-        level_features(xs, g.n, g.nnz, g.nwords, F, mf, nzw, reached, redges, sel.level,
-                       sel.prev_choice, st.ctr->frontier);
-    __syncthreads();
-    const int choice = sel.params ? mlp_argmin(s_params, xs, h1, h2, ys, sel.valid_mask, &s_choice) : -1;
-    if (tid == 0) {
+    const unsigned long long reached = c.reached + F;
+    const unsigned long long redges = c.reached_edges + mf;
+    level_features(xs, g.n, g.nnz, g.nwords, F, mf, nzw, reached, redges, sel.level, sel.prev_choice,
+                   c.frontier);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int choice = sel.params ? mlp_argmin_wave(s_params, xs, h1, h2, ys, sel.valid_mask) : -1;
+    if (lane == 0) {
+        c.frontier = F;
+        c.frontier_edges = mf;
+        c.frontier_words = nzw;
+        c.scanned = scn;
+        c.words = wds;
+        c.bypassed = byp;
+        c.reached = reached;
+        c.reached_edges = redges;
         const int heur = heuristic_choice(F, g.n);
-        st.ctr->frontier = F;
-        st.ctr->frontier_edges = mf;
-        st.ctr->frontier_words = nzw;
-        st.ctr->scanned = scn;
-        st.ctr->words = wds;
-        st.ctr->bypassed = byp;
-        st.ctr->reached = reached;
-        st.ctr->reached_edges = redges;
-        st.ctr->heur_choice = heur;
-        st.ctr->sel_choice = choice >= 0 ? choice : heur;
-        for (int i = 0; i < ALBBA_MLP_IN; ++i) st.ctr->features[i] = xs[i];
-        if (mail) {
-            // the whole block to host memory, then the sequence number with a
-            // system-scope release: the host sees the data once it sees seq
-            const unsigned long long *src = reinterpret_cast<const unsigned long long *>(st.ctr);
-            unsigned long long *dst = reinterpret_cast<unsigned long long *>(mail);
-            const int words = (int)(offsetof(albd_counters, seq) / 8);
-            for (int i = 0; i < words; ++i) dst[i] = src[i];
-            __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        c.heur_choice = heur;
+        c.sel_choice = choice >= 0 ? choice : heur;
+        for (int i = 0; i < ALBBA_MLP_IN; ++i) c.features[i] = xs[i];
         // hand the level's lists over to the next level: a pushed level's
         // queue becomes the queue (after a pull the queue is rebuilt from
         // the bitmap, from zero); a used unvisited-word list becomes the list
-        if (!sel.first)
-            st.ctr->queue_len = sel.prev_choice == ALBBA_CHOICE_PUSH ? st.ctr->qnext_len : 0ull;
-        st.ctr->qnext_len = 0;
+        if (!sel.first) c.queue_len = sel.prev_choice == ALBBA_CHOICE_PUSH ? c.qnext_len : 0ull;
+        c.qnext_len = 0;
         if (sel.prev_choice == ALBBA_CHOICE_PULL_SB || sel.prev_choice == ALBBA_CHOICE_PULL_CB)
-            st.ctr->active_len = st.ctr->active_next;
-        st.ctr->active_next = 0;
+            c.active_len = c.active_next;
+        c.active_next = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < kCtrWords) {
+        const unsigned long long w = s_ctr.w[lane];
+        ctr_words[lane] = w;
+        if (mail) reinterpret_cast<unsigned long long *>(mail)[lane] = w;
+    }
+    if (mail) {
+        // the whole block to host memory, then the sequence number with a
+        // system-scope release: the host sees the data once it sees seq
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

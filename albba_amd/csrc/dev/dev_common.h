@@ -46,12 +46,48 @@ __device__ __forceinline__ T wave_exclusive_scan(T v, T *total)
     return x - v;
 }
 
+// One DPP move of a 32-bit value across lanes (CTRL = the DPP pattern):
+// lanes whose source lies outside the pattern, or whose row is not in
+// ROW_MASK, read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint64_t dpp_or_zero(uint64_t v)
+{
+    const uint32_t lo = dpp_or_zero<CTRL, ROW_MASK>((uint32_t)v);
+    const uint32_t hi = dpp_or_zero<CTRL, ROW_MASK>((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane; call with the
+// whole wave active.  DPP moves inside rows of 16 lanes (quad_perm 1,0,3,2;
+// quad_perm 2,3,0,1; row_shr 4; row_shr 8 leave each row's total in its lane
+// 15), then row_bcast 15 into rows 1 and 3 and row_bcast 31 into rows 2 and
+// 3 leave the wave's total in lane 63.  A shuffle-based tree costs a
+// dependent LDS round trip per step and per 32-bit half instead.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit integers");
+    using U = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+    U x = (U)v;
+    x += dpp_or_zero<0xb1>(x);
+    x += dpp_or_zero<0x4e>(x);
+    x += dpp_or_zero<0x114>(x);
+    x += dpp_or_zero<0x118>(x);
+    x += dpp_or_zero<0x142, 0xa>(x);
+    x += dpp_or_zero<0x143, 0xc>(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+    if constexpr (sizeof(T) == 8) {
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), 63);
+        return (T)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (T)lo;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -30,6 +30,10 @@ def check(dg, rp, col, root, policy=A.SWITCH_HEURISTIC, fixed=0, **kw):
     np.testing.assert_array_equal(o["parent"], pa)
     np.testing.assert_array_equal(o["frontier"], np.bincount(lv[lv >= 0]))
     assert o["reached"] == int((lv >= 0).sum())
+    # out-edges of the reached vertices (a vertex's push out-degree is the
+    # number of rows listing it; equal to its row length when symmetric)
+    deg = np.bincount(np.asarray(col), minlength=len(lv))
+    assert o["edges_in_component"] == int(deg[lv >= 0].sum())
     assert O.validate(rp, col, int(root), o["level"], o["parent"]) == 0
     return o
 
