@@ -103,6 +103,11 @@ class _BatchResult(C.Structure):
                 ("team", C.c_int32)]
 
 
+class _BuildReport(C.Structure):
+    _fields_ = [("upload_ms", C.c_float), ("build_ms", C.c_float), ("rows_sorted", C.c_int32),
+                ("reserved_", C.c_int32)]
+
+
 class _TuneConfig(C.Structure):
     _fields_ = [("C", C.c_int32), ("sigma", C.c_int32), ("mapping", C.c_int32),
                 ("status", C.c_int32), ("ms", C.c_double), ("batch_ms", C.c_double)]
@@ -142,6 +147,9 @@ def lib() -> C.CDLL:
         "albba_ctx_synchronize": (C.c_int, [C.c_void_p]),
         "albba_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_float)]),
         "albba_graph_upload": (C.c_int, [C.c_void_p, C.POINTER(_SELL), C.POINTER(C.c_void_p)]),
+        "albba_graph_build": (C.c_int, [C.c_void_p, C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_void_p), C.POINTER(_BuildReport)]),
+        "albba_graph_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.POINTER(_SELL))]),
         "albba_graph_free": (None, [C.c_void_p]),
         "albba_graph_device_bytes": (C.c_int64, [C.c_void_p]),
         "albba_bfs_opts_default": (None, [C.POINTER(_Opts)]),
@@ -392,12 +400,38 @@ class Context:
 
 class DeviceGraph:
     def __init__(self, ctx: Context, sell: SellCS):
+        """Upload a layout built on the host (Graph.sellcs)."""
         self.ctx = ctx
         self.n = sell.raw.n
         self.nnz = int(sell.raw.nnz)
+        self.build_report = None
         self._p = C.c_void_p()
         _check(lib().albba_graph_upload(ctx._p, sell._p, C.byref(self._p)))
         ctx._adopt(self)
+
+    @classmethod
+    def build(cls, ctx: Context, graph: "Graph", C_: int = 64, sigma: int = 4096,
+              pad: int = 1) -> "DeviceGraph":
+        """Build the layout on the device from the CSR (albba_graph_build):
+        the same device arrays as DeviceGraph(ctx, graph.sellcs(C_, sigma, pad))."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = graph.n
+        self.nnz = int(graph.nnz)
+        self._p = C.c_void_p()
+        rep = _BuildReport()
+        _check(lib().albba_graph_build(ctx._p, C.byref(graph._raw), C_, sigma, pad, C.byref(self._p),
+                                       C.byref(rep)))
+        self.build_report = {"upload_ms": float(rep.upload_ms), "build_ms": float(rep.build_ms),
+                             "rows_sorted": bool(rep.rows_sorted)}
+        ctx._adopt(self)
+        return self
+
+    def layout(self) -> "SellCS":
+        """The device layout copied to the host (albba_graph_layout)."""
+        p = C.POINTER(_SELL)()
+        _check(lib().albba_graph_layout(self.ctx._p, self._p, C.byref(p)))
+        return SellCS(p, None)
 
     @property
     def device_bytes(self) -> int:

@@ -153,6 +153,29 @@ ALBD_HIDDEN int albd_fill_u64(albd_ctx *c, void *p, uint64_t v, size_t count);
 /* dst[i] = src[i] for count 64-bit words (16-byte aligned pointers) */
 ALBD_HIDDEN int albd_copy_u64(albd_ctx *c, void *dst, const void *src, size_t count);
 
+/* ---- SELL-C-sigma build on the device ---- */
+typedef struct albd_build_input {
+    const int64_t *row_ptr; /* host CSR (validated by the caller: row_ptr[0] = 0,
+                               non-decreasing, row_ptr[n] = nnz)              */
+    const int32_t *col;
+    int64_t nnz;
+    int32_t n, C, sigma, pad;
+    int32_t symmetric;      /* 1: the push CSR is the layout's own rows      */
+    int32_t pad_;
+} albd_build_input;
+typedef struct albd_build_report {
+    float upload_ms;        /* CSR host -> device                            */
+    float build_ms;         /* every build step on the device                */
+    int32_t rows_sorted;    /* 1: every row was already in ascending order   */
+    int32_t pad_;
+} albd_build_report;
+/* device allocation for the graph's own arrays (the caller owns them) */
+typedef void *(*albd_build_alloc)(void *arg, size_t bytes);
+/* Fill every layout array of *g (row order, positions, slices, columns, the
+ * push and pull CSRs) from the CSR; synchronous.  Temporaries are freed. */
+ALBD_HIDDEN int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_graph *g,
+                                 albd_build_alloc alloc, void *alloc_arg, albd_build_report *rep);
+
 /* ---- BFS pieces ---- */
 /* haslen[w] from row_len */
 ALBD_HIDDEN int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen);

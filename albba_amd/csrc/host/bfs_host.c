@@ -174,6 +174,53 @@ void albba_graph_free(albba_graph *G)
     free(G);
 }
 
+/* The BFS state of a graph whose layout arrays are on the device (shapes
+ * from G->g): allocate, build haslen, fill the keys, and keep the host copy
+ * of pos_of.  Sets the error message on failure. */
+static int graph_state_alloc(albba_graph *G)
+{
+    albd_graph *g = &G->g;
+    const int64_t nwords = g->nwords, npos = g->npos;
+    const int32_t n = g->n;
+    int rc = 0;
+    g->haslen = (const uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    albd_state *st = &G->st;
+    const int64_t nsum = (nwords + 63) / 64 + 1;
+    st->visited = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->front = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->next = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    st->summary = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nsum, &rc);
+    st->queue = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->queue_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
+    st->prefix = (int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
+    st->block_sums = (int64_t *)dalloc(G, sizeof(int64_t) * (ALBD_SCAN_BLOCKS + 1), &rc);
+    st->pmin = (uint32_t *)dalloc(G, sizeof(uint32_t) * (size_t)npos, &rc);
+    st->lp = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
+    st->level = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)n, &rc);
+    st->parent = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)n, &rc);
+    st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
+    st->active_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
+    st->ctr = (albd_counters *)dalloc(G, sizeof(albd_counters), &rc);
+    st->slots = (albd_slot *)dalloc(G, sizeof(albd_slot) * ALBD_NSLOT, &rc);
+    st->level_log = (int64_t *)dalloc(G, sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS, &rc);
+    st->pkey = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
+    st->persist = (albd_persist *)dalloc(G, sizeof(albd_persist), &rc);
+    G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
+    G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
+    if (rc) return albba_fail(rc, "%s", albd_error());
+    if ((rc = albd_build_haslen(G->ctx->d, g, (uint64_t *)g->haslen)) ||
+        (rc = albd_fill_u64(G->ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
+        (rc = albd_fill_u32(G->ctx->d, st->pmin, 0xFFFFFFFFu, (size_t)npos)) ||
+        (rc = albd_sync(G->ctx->d)))
+        return albba_fail(rc, "%s", albd_error());
+    G->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
+    G->host_log = (int64_t *)albba_malloc(sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS);
+    if (!G->pos_of || !G->host_log) return albba_fail(ALBBA_ENOMEM, "host arrays");
+    if ((rc = albd_d2h(G->ctx->d, G->pos_of, g->pos_of, sizeof(int32_t) * (size_t)n)))
+        return albba_fail(rc, "%s", albd_error());
+    return ALBBA_OK;
+}
+
 int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
 {
     if (!ctx || !s || !out) return albba_fail(ALBBA_EINVAL, "graph_upload: null argument");
@@ -181,6 +228,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     if (!G) return albba_fail(ALBBA_ENOMEM, "graph_upload");
     G->ctx = ctx;
     G->n = s->n;
+    G->sigma = s->sigma;
     G->nnz = s->nnz;
     const int64_t npos_raw = (int64_t)s->nslices * s->C;
     const int64_t nwords = (npos_raw + 63) / 64;
@@ -214,36 +262,11 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         g->pull_ptr = g->push_ptr;
         g->pull_col = g->push_col;
     }
-    g->haslen = (const uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     g->pos_of = (const int32_t *)dalloc(G, sizeof(int32_t) * (size_t)(s->n > 0 ? s->n : 1), &rc);
-    albd_state *st = &G->st;
-    const int64_t nsum = (nwords + 63) / 64 + 1;
-    st->visited = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
-    st->front = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
-    st->next = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
-    st->summary = (uint64_t *)dalloc(G, sizeof(uint64_t) * (size_t)nsum, &rc);
-    st->queue = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
-    st->queue_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)npos, &rc);
-    st->prefix = (int64_t *)dalloc(G, sizeof(int64_t) * (size_t)(npos + 1), &rc);
-    st->block_sums = (int64_t *)dalloc(G, sizeof(int64_t) * (ALBD_SCAN_BLOCKS + 1), &rc);
-    st->pmin = (uint32_t *)dalloc(G, sizeof(uint32_t) * (size_t)npos, &rc);
-    st->lp = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
-    st->level = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
-    st->parent = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)s->n, &rc);
-    st->active = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
-    st->active_next = (int32_t *)dalloc(G, sizeof(int32_t) * (size_t)nwords, &rc);
-    st->ctr = (albd_counters *)dalloc(G, sizeof(albd_counters), &rc);
-    st->slots = (albd_slot *)dalloc(G, sizeof(albd_slot) * ALBD_NSLOT, &rc);
-    st->level_log = (int64_t *)dalloc(G, sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS, &rc);
-    st->pkey = (unsigned long long *)dalloc(G, sizeof(unsigned long long) * (size_t)npos, &rc);
-    st->persist = (albd_persist *)dalloc(G, sizeof(albd_persist), &rc);
-    G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
-    G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
     if (rc) {
         int code = rc;
-        const char *msg = albd_error();
         char buf[512];
-        snprintf(buf, sizeof buf, "%s", msg);
+        snprintf(buf, sizeof buf, "%s", albd_error());
         albba_graph_free(G);
         return albba_fail(code, "graph_upload: %s", buf);
     }
@@ -259,21 +282,147 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
         (rc = albd_h2d(ctx->d, (void *)g->push_col, s->push_col, sizeof(int32_t) * (size_t)s->nnz)) ||
         (s->pull_ptr && (rc = upload_padded_i64(ctx, (int64_t *)g->pull_ptr, s->pull_ptr, npos_raw + 1, npos + 1))) ||
         (s->pull_ptr && (rc = albd_h2d(ctx->d, (void *)g->pull_col, s->pull_col, sizeof(int32_t) * (size_t)s->nnz))) ||
-        (rc = albd_h2d(ctx->d, (void *)g->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n)) ||
-        (rc = albd_build_haslen(ctx->d, g, (uint64_t *)g->haslen)) ||
-        (rc = albd_fill_u64(ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
-        (rc = albd_fill_u32(ctx->d, st->pmin, 0xFFFFFFFFu, (size_t)npos)) ||
-        (rc = albd_sync(ctx->d))) {
+        (rc = albd_h2d(ctx->d, (void *)g->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n))) {
         char buf[512];
         snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "host staging buffer" : albd_error());
         albba_graph_free(G);
         return albba_fail(rc, "graph_upload: %s", buf);
     }
-    G->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)s->n);
-    G->host_log = (int64_t *)albba_malloc(sizeof(int64_t) * 4 * ALBD_PERSIST_LEVELS);
-    if (!G->pos_of || !G->host_log) { albba_graph_free(G); return albba_fail(ALBBA_ENOMEM, "graph_upload: host arrays"); }
-    memcpy(G->pos_of, s->pos_of, sizeof(int32_t) * (size_t)s->n);
+    if ((rc = graph_state_alloc(G))) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", albba_last_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_upload: %s", buf);
+    }
     *out = G;
+    return ALBBA_OK;
+}
+
+static void *build_alloc(void *arg, size_t bytes)
+{
+    int rc = 0;
+    void *p = dalloc((albba_graph *)arg, bytes, &rc);
+    return rc ? NULL : p;
+}
+
+int albba_graph_build(albba_ctx *ctx, const albba_csr *cg, int C, int sigma, int pad, albba_graph **out,
+                      albba_build_report *report)
+{
+    if (!ctx || !cg || !out) return albba_fail(ALBBA_EINVAL, "graph_build: null argument");
+    if (C != 16 && C != 32 && C != 64 && C != 128)
+        return albba_fail(ALBBA_EINVAL, "graph_build: C must be 16, 32, 64 or 128 (got %d)", C);
+    if (pad != 1 && pad != 16 && pad != 32)
+        return albba_fail(ALBBA_EINVAL, "graph_build: pad must be 1, 16 or 32 (got %d)", pad);
+    if (cg->n < 1 || !cg->row_ptr || (cg->nnz > 0 && !cg->col))
+        return albba_fail(ALBBA_EINVAL, "graph_build: empty graph");
+    if (sigma < 1 || (sigma % C != 0 && sigma < cg->n))
+        return albba_fail(ALBBA_EINVAL, "graph_build: sigma (%d) must be a multiple of C (%d) or >= n", sigma, C);
+    if (cg->nnz < 0 || cg->nnz > INT32_MAX) return albba_fail(ALBBA_ERANGE, "graph_build: nnz beyond int32");
+    /* the kernels index col through row_ptr: check it before the upload */
+    if (cg->row_ptr[0] != 0 || cg->row_ptr[cg->n] != cg->nnz)
+        return albba_fail(ALBBA_EINVAL, "graph_build: row_ptr must run from 0 to nnz");
+    int monotone = 1;
+#pragma omp parallel for schedule(static) reduction(&& : monotone)
+    for (int32_t i = 0; i < cg->n; ++i)
+        if (cg->row_ptr[i + 1] < cg->row_ptr[i]) monotone = 0;
+    if (!monotone) return albba_fail(ALBBA_EINVAL, "graph_build: row_ptr decreases");
+    albba_graph *G = (albba_graph *)albba_calloc(1, sizeof(albba_graph));
+    if (!G) return albba_fail(ALBBA_ENOMEM, "graph_build");
+    G->ctx = ctx;
+    G->n = cg->n;
+    G->nnz = cg->nnz;
+    G->sigma = sigma;
+    albd_build_input in;
+    memset(&in, 0, sizeof in);
+    in.row_ptr = cg->row_ptr;
+    in.col = cg->col;
+    in.nnz = cg->nnz;
+    in.n = cg->n;
+    in.C = C;
+    in.sigma = sigma;
+    in.pad = pad;
+    in.symmetric = cg->symmetric == 1;
+    albd_build_report r;
+    memset(&r, 0, sizeof r);
+    int rc = albd_build_graph(ctx->d, &in, &G->g, build_alloc, G, &r);
+    if (rc) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", albd_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_build: %s", buf);
+    }
+    if ((rc = graph_state_alloc(G))) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", albba_last_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_build: %s", buf);
+    }
+    if (report) {
+        report->upload_ms = r.upload_ms;
+        report->build_ms = r.build_ms;
+        report->rows_sorted = r.rows_sorted;
+        report->reserved_ = 0;
+    }
+    *out = G;
+    return ALBBA_OK;
+}
+
+int albba_graph_layout(albba_ctx *ctx, const albba_graph *G, albba_sellcs **out)
+{
+    if (!ctx || !G || !out || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "graph_layout: bad argument");
+    const albd_graph *g = &G->g;
+    const int64_t ns = g->nslices, npos_raw = ns * g->C;
+    albba_sellcs *s = (albba_sellcs *)albba_calloc(1, sizeof(albba_sellcs));
+    if (!s) return albba_fail(ALBBA_ENOMEM, "graph_layout");
+    s->n = g->n;
+    s->C = g->C;
+    s->sigma = G->sigma;
+    s->pad = g->pad;
+    s->nslices = g->nslices;
+    s->nnz = g->nnz;
+    s->padded_nnz = g->padded_nnz;
+    s->nvops = g->nvops;
+    const int directed = g->pull_ptr != g->push_ptr;
+    s->row_order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)g->n);
+    s->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)g->n);
+    s->slice_widths = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(ns > 0 ? ns : 1));
+    s->slice_pointers = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)(ns + 1));
+    s->vop_pointers = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)(ns + 1));
+    s->vop_lengths = (uint8_t *)albba_malloc((size_t)(g->nvops > 0 ? g->nvops : 1));
+    s->column_indices = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(g->padded_nnz > 0 ? g->padded_nnz : 1));
+    s->row_lengths = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)npos_raw);
+    s->push_ptr = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)(npos_raw + 1));
+    s->push_col = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(g->nnz > 0 ? g->nnz : 1));
+    if (directed) {
+        s->pull_ptr = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)(npos_raw + 1));
+        s->pull_col = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(g->nnz > 0 ? g->nnz : 1));
+    }
+    if (!s->row_order || !s->pos_of || !s->slice_widths || !s->slice_pointers || !s->vop_pointers ||
+        !s->vop_lengths || !s->column_indices || !s->row_lengths || !s->push_ptr || !s->push_col ||
+        (directed && (!s->pull_ptr || !s->pull_col))) {
+        albba_sellcs_free(s);
+        return albba_fail(ALBBA_ENOMEM, "graph_layout");
+    }
+    albd_ctx *d = ctx->d;
+    int rc = 0;
+    if ((rc = albd_d2h(d, s->row_order, g->row_order, sizeof(int32_t) * (size_t)g->n)) ||
+        (rc = albd_d2h(d, s->pos_of, g->pos_of, sizeof(int32_t) * (size_t)g->n)) ||
+        (rc = albd_d2h(d, s->slice_widths, g->slice_width, sizeof(int32_t) * (size_t)ns)) ||
+        (rc = albd_d2h(d, s->slice_pointers, g->slice_ptr, sizeof(int64_t) * (size_t)(ns + 1))) ||
+        (rc = albd_d2h(d, s->vop_pointers, g->vop_ptr, sizeof(int64_t) * (size_t)(ns + 1))) ||
+        (rc = albd_d2h(d, s->vop_lengths, g->vop_len, (size_t)g->nvops)) ||
+        (rc = albd_d2h(d, s->column_indices, g->colidx, sizeof(int32_t) * (size_t)g->padded_nnz)) ||
+        (rc = albd_d2h(d, s->row_lengths, g->row_len, sizeof(int32_t) * (size_t)npos_raw)) ||
+        (rc = albd_d2h(d, s->push_ptr, g->push_ptr, sizeof(int64_t) * (size_t)(npos_raw + 1))) ||
+        (rc = albd_d2h(d, s->push_col, g->push_col, sizeof(int32_t) * (size_t)g->nnz)) ||
+        (directed && (rc = albd_d2h(d, s->pull_ptr, g->pull_ptr, sizeof(int64_t) * (size_t)(npos_raw + 1)))) ||
+        (directed && (rc = albd_d2h(d, s->pull_col, g->pull_col, sizeof(int32_t) * (size_t)g->nnz)))) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", albd_error());
+        albba_sellcs_free(s);
+        return albba_fail(rc, "graph_layout: %s", buf);
+    }
+    *out = s;
     return ALBBA_OK;
 }
 

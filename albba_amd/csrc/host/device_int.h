@@ -16,6 +16,7 @@ struct albba_graph {
     albd_graph g;
     albd_state st;
     int32_t n;
+    int32_t sigma;       /* the layout's sigma as requested */
     int64_t nnz;
     int32_t *pos_of;     /* host: vertex id -> position */
     int64_t device_bytes;

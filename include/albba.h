@@ -289,6 +289,24 @@ ALBBA_API int albba_ctx_synchronize(albba_ctx *ctx);
 /* Upload a SELL layout (+ push CSR) and allocate the BFS state on the
  * context's device. */
 ALBBA_API int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out);
+/* Build the layout on the device from a host CSR -- row lengths, the
+ * sigma-window sort, slices, active lanes, padded columns, the push and pull
+ * CSRs -- and allocate the BFS state: the same device arrays, bit for bit, as
+ * albba_sellcs_build followed by albba_graph_upload (same arguments and
+ * errors; rows out of ascending order are sorted on the device).  The
+ * reference builds on the host, timed as conv_time (bfs/src/bfs_SELLCSv2.c:
+ * 34-59 sellcs_preprocess, called at :161-168).  report may be NULL. */
+typedef struct albba_build_report {
+    float upload_ms;     /* CSR host -> device copy                          */
+    float build_ms;      /* every build step on the device                   */
+    int32_t rows_sorted; /* 1: the rows were already in ascending id order  */
+    int32_t reserved_;
+} albba_build_report;
+ALBBA_API int albba_graph_build(albba_ctx *ctx, const albba_csr *g, int C, int sigma, int pad,
+                                albba_graph **out, albba_build_report *report);
+/* Copy a device graph's layout to the host as an albba_sellcs (free with
+ * albba_sellcs_free); row_lengths and push_ptr cover nslices * C positions. */
+ALBBA_API int albba_graph_layout(albba_ctx *ctx, const albba_graph *g, albba_sellcs **out);
 ALBBA_API void albba_graph_free(albba_graph *g);
 ALBBA_API int64_t albba_graph_device_bytes(const albba_graph *g);
 

@@ -1,0 +1,111 @@
+"""The SELL-C-sigma layout built on the device (albba_graph_build) against
+the host builder (albba_sellcs_build, itself pinned to the reference
+builder's golden arrays): every array bit for bit, for every slice height,
+sigma and column padding, symmetric and directed graphs, rows out of order;
+then BFS on the device-built graph against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import albba_amd as A
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GG = np.load(os.path.join(HERE, "golden", "golden_graphs.npz"))
+GOLDEN = sorted({k.split("__")[0] for k in GG.files})
+
+
+def assert_same_layout(ctx, g, C_, sigma, pad):
+    host = g.sellcs(C_, sigma, pad).arrays()
+    dg = A.DeviceGraph.build(ctx, g, C_, sigma, pad)
+    dev = dg.layout().arrays()
+    for k, v in host.items():
+        if v is None:
+            assert dev[k] is None, k
+        else:
+            np.testing.assert_array_equal(dev[k], v, err_msg=f"{k} C={C_} sigma={sigma} pad={pad}")
+    return dg
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_graphs_every_shape(ctx, name):
+    rp, col = GG[f"{name}__rp"], GG[f"{name}__col"]
+    g = A.Graph.from_arrays(rp, col, symmetric=not name.startswith("dir_"))
+    for C_ in (16, 32, 64, 128):
+        for sigma in sorted({C_, 256, 4096, g.n}):
+            if sigma % C_ and sigma < g.n:
+                continue
+            for pad in (1, 16, 32):
+                assert_same_layout(ctx, g, C_, sigma, pad).close()
+
+
+@pytest.mark.parametrize("kind", ["rmat", "er", "grid", "directed"])
+def test_medium_graphs_and_bfs(ctx, kind):
+    if kind == "directed":
+        base = A.Graph.rmat(13, 8, seed=91, perm_seed=92)
+        rp, col = base.row_ptr, base.col
+        # keep the entries u -> v with u < v of a symmetric graph: not symmetric
+        rows = np.repeat(np.arange(base.n), np.diff(rp))
+        keep = col > rows
+        rp2 = np.zeros(base.n + 1, np.int64)
+        np.cumsum(np.bincount(rows[keep], minlength=base.n), out=rp2[1:])
+        g = A.Graph.from_arrays(rp2, col[keep].astype(np.int32), symmetric=False)
+    else:
+        g = {"rmat": lambda: A.Graph.rmat(14, 16, seed=93, perm_seed=94),
+             "er": lambda: A.Graph.er(30000, 6.0, seed=95, perm_seed=96),
+             "grid": lambda: A.Graph.grid(120, 130, 0.1, seed=97, perm_seed=98)}[kind]()
+    for C_, sigma, pad in [(64, 4096, 1), (32, 256, 16), (128, g.n, 32), (16, 16, 1)]:
+        dg = assert_same_layout(ctx, g, C_, sigma, pad)
+        if (C_, sigma) == (64, 4096):
+            assert dg.build_report["rows_sorted"]
+            rp, col = g.row_ptr, g.col
+            for root in g.pick_roots(3, 99):
+                o = dg.bfs(int(root))
+                lv, pa = O.bfs_queue(rp, col, int(root))
+                np.testing.assert_array_equal(o["level"], lv)
+                np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+
+
+def test_rows_out_of_order_are_sorted(ctx):
+    base = A.Graph.rmat(12, 16, seed=101, perm_seed=102)
+    rp, col = base.row_ptr.copy(), base.col.copy()
+    rng = np.random.default_rng(103)
+    for v in range(base.n):
+        rng.shuffle(col[rp[v]:rp[v + 1]])
+    g = A.Graph.from_arrays(rp, col, symmetric=True)
+    dg = assert_same_layout(ctx, g, 64, 4096, 1)
+    assert not dg.build_report["rows_sorted"]
+    root = int(base.pick_roots(1, 104)[0])
+    o = dg.bfs(root)
+    lv, pa = O.bfs_queue(base.row_ptr, base.col, root)
+    np.testing.assert_array_equal(o["level"], lv)
+    np.testing.assert_array_equal(o["parent"], pa)
+    dg.close()
+
+
+def test_empty_rows_and_tiny_graphs(ctx):
+    # isolated vertices only; then one edge among many isolated vertices
+    for rp, col in [(np.zeros(101, np.int64), np.zeros(0, np.int32)),
+                    (np.r_[0, 1, 2, np.full(98, 2)].astype(np.int64), np.array([1, 0], np.int32))]:
+        g = A.Graph.from_arrays(rp, col, symmetric=True)
+        for C_, sigma, pad in [(64, 4096, 1), (16, 16, 32)]:
+            assert_same_layout(ctx, g, C_, sigma, pad).close()
+
+
+def test_build_errors(ctx):
+    g = A.Graph.rmat(8, 4, seed=105, perm_seed=106)
+    for C_, sigma, pad in [(48, 4096, 1), (64, 100, 1), (64, 4096, 7)]:
+        with pytest.raises(A.AlbbaError):
+            A.DeviceGraph.build(ctx, g, C_, sigma, pad)
+    bad = g.col.copy()
+    bad[5] = g.n  # a column index outside [0, n)
+    with pytest.raises(A.AlbbaError, match="column index"):
+        A.DeviceGraph.build(ctx, A.Graph.from_arrays(g.row_ptr, bad, symmetric=True), 64, 4096, 1)
+    rp = g.row_ptr.copy()
+    rp[3], rp[4] = rp[4], rp[3]  # decreasing row pointers
+    if rp[3] != rp[4]:
+        with pytest.raises(A.AlbbaError, match="row_ptr"):
+            A.DeviceGraph.build(ctx, A.Graph.from_arrays(rp, g.col, symmetric=True), 64, 4096, 1)
