@@ -16,9 +16,10 @@
 //   5. per (slice, column): active lanes (a binary search in the slice's
 //      descending lengths) and the padded column size, whose scan gives each
 //      column's start and the slice pointers;
-//   6. the columns: a wave takes 64 consecutive positions and spreads their
-//      entries evenly over its lanes; the row-major copy (push CSR when the
-//      matrix is symmetric, pull CSR otherwise) is written in the same pass;
+//   6. the row-major copy of the rows with columns as positions (the push
+//      CSR when the matrix is symmetric, the pull CSR otherwise), written
+//      coalesced, then the column-major layout from it: lanes are a slice's
+//      rows, so each column is one coalesced store;
 //   7. directed graphs: the push CSR (transpose over positions) by a degree
 //      histogram, a scan, an atomic fill and a per-row sort.
 #include <rocprim/device/device_radix_sort.hpp>
@@ -67,43 +68,52 @@ __device__ __forceinline__ void wave_rows_foreach(int32_t len, F &&f)
     }
 }
 
-// 1. lengths, longest row, checks
-__global__ __launch_bounds__(kBB) void k_lengths(const int64_t *rp, const int32_t *col, int32_t n,
-                                                 int32_t *len, unsigned int *flags)
+// 1. lengths, longest row, checks.  Rows are in ascending order exactly
+// when every descent col[k-1] > col[k] sits at the start of a row: one flat
+// pass counts all descents (and checks the column range), one pass over the
+// rows counts the descents at row starts.  flags: [0] longest row, [1] a
+// column index outside [0, n), [2] descents, [3] descents at row starts.
+__global__ __launch_bounds__(kBB) void k_rows(const int64_t *rp, const int32_t *col, int32_t n, int64_t nnz,
+                                              int32_t *len, unsigned long long *flags)
 {
-    // flags[0]: longest row; flags[1]: a row out of ascending order;
-    // flags[2]: a column index outside [0, n)
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
-    unsigned int mx = 0, unsorted = 0, bad = 0;
-    for (int64_t r0 = wave * 64; r0 < n; r0 += nwaves * 64) {
-        const int64_t r = r0 + lane;
-        int64_t start = 0;
-        int32_t l = 0;
-        if (r < n) {
-            start = rp[r];
-            l = (int32_t)(rp[r + 1] - start);
-            len[r] = l;
-            mx = max(mx, (unsigned int)l);
-        }
-        wave_rows_foreach(l, [&](int i, int64_t j, bool act) {
-            const int64_t s = __shfl(start, i, 64);
-            if (act) {
-                const int32_t c = col[s + j];
-                if (c < 0 || c >= n) bad = 1;
-                if (j > 0 && col[s + j - 1] > c) unsorted = 1;
-            }
-        });
+    unsigned long long mx = 0, at_start = 0;
+    for (int64_t r = (int64_t)blockIdx.x * kBB + threadIdx.x; r < n; r += (int64_t)gridDim.x * kBB) {
+        const int64_t a = rp[r], b = rp[r + 1];
+        len[r] = (int32_t)(b - a);
+        mx = max(mx, (unsigned long long)(b - a));
+        // a non-empty row's start is distinct from every other one's
+        if (a > 0 && b > a && col[a - 1] > col[a]) ++at_start;
     }
-    const uint64_t any_unsorted = __ballot(unsorted != 0), any_bad = __ballot(bad != 0);
-    // wave maximum by butterfly
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) mx = max(mx, (unsigned int)__shfl_xor(mx, d, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 32, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 16, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 8, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 4, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 2, 64));
+    mx = max(mx, (unsigned long long)__shfl_xor(mx, 1, 64));
+    at_start = wave_sum(at_start);
     if (lane == 0) {
         if (mx) atomicMax(&flags[0], mx);
-        if (any_unsorted) atomicOr(&flags[1], 1u);
-        if (any_bad) atomicOr(&flags[2], 1u);
+        if (at_start) atomicAdd(&flags[3], at_start);
+    }
+    (void)nnz;
+}
+
+__global__ __launch_bounds__(kBB) void k_entries(const int32_t *col, int32_t n, int64_t nnz,
+                                                 unsigned long long *flags)
+{
+    const int lane = lane_id();
+    unsigned long long desc = 0, bad = 0;
+    for (int64_t k = (int64_t)blockIdx.x * kBB + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBB) {
+        const int32_t c = col[k];
+        if (c < 0 || c >= n) bad = 1;
+        if (k > 0 && col[k - 1] > c) ++desc;
+    }
+    desc = wave_sum(desc);
+    const uint64_t any_bad = __ballot(bad != 0);
+    if (lane == 0) {
+        if (desc) atomicAdd(&flags[2], desc);
+        if (any_bad) atomicOr(&flags[1], 1ull);
     }
 }
 
@@ -164,7 +174,7 @@ __global__ __launch_bounds__(kBB) void k_widths(const int32_t *row_len, int32_t 
 // 5. active lanes per column and padded column sizes
 __global__ __launch_bounds__(kBB) void k_columns(const int32_t *row_len, const int64_t *vop_ptr, int64_t ns,
                                                  int32_t C, int32_t pad, int64_t nvops, uint8_t *vop_len,
-                                                 int64_t *csize)
+                                                 int64_t *csize, int32_t *slice_of)
 {
     for (int64_t q = (int64_t)blockIdx.x * kBB + threadIdx.x; q <= nvops; q += (int64_t)gridDim.x * kBB) {
         if (q == nvops) {
@@ -189,6 +199,7 @@ __global__ __launch_bounds__(kBB) void k_columns(const int32_t *row_len, const i
         }
         vop_len[q] = (uint8_t)(a - 1);
         csize[q] = pad_up(a, pad);
+        slice_of[q] = (int32_t)lo;
     }
 }
 
@@ -199,46 +210,155 @@ __global__ __launch_bounds__(kBB) void k_slice_ptr(const int64_t *vop_ptr, const
         slice_ptr[s] = cstart[vop_ptr[s]];
 }
 
-struct FillArgs {
+// rows longer than this are copied by the long-row pass, a block per row
+constexpr int32_t kLongRow = 4096;
+
+struct RowArgs {
     const int64_t *rp;
     const int32_t *col;
     const int32_t *row_order, *row_len, *pos_of;
-    const int64_t *vop_ptr, *cstart;
-    const int64_t *rm_ptr; // row-major copy: pointers over positions
-    int32_t *colidx;
+    const int64_t *rm_ptr;  // row-major copy: pointers over positions
     int32_t *rm_col;
     unsigned int *push_cnt; // directed: out-degree histogram over positions, or null
     int32_t n;
-    int32_t C;
-    int32_t log2C;
 };
 
-// 6. columns and the row-major copy
-__global__ __launch_bounds__(kBB) void k_fill(FillArgs a)
+__device__ __forceinline__ void copy_entry(const RowArgs &a, int64_t src, int64_t dst)
 {
+    const int32_t c = a.pos_of[a.col[src]];
+    a.rm_col[dst] = c;
+    if (a.push_cnt) atomicAdd(&a.push_cnt[c], 1u);
+}
+
+// 6a. the row-major copy (columns as positions).  A wave takes 64
+// consecutive positions and spreads their entries evenly over its lanes, so
+// that neighbouring lanes read neighbouring entries of one row (a lane
+// walking a row of its own would have every lane of every resident wave on a
+// different line).  Four steps of 64 entries are handled together so that
+// their position gathers are in flight at once; the streamed reads of the
+// CSR columns and the stores of the copy are non-temporal, which leaves the
+// caches to the gathered position table.  Rows longer than kLongRow are left
+// to k_rowmajor_long.
+__global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
+{
+    constexpr int U = 4;
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
     for (int64_t p0 = wave * 64; p0 < a.n; p0 += nwaves * 64) {
         const int64_t p = p0 + lane;
-        int64_t start = 0, vp = 0, rm = 0;
+        int64_t start = 0, rm = 0;
         int32_t l = 0;
         if (p < a.n) {
-            start = a.rp[a.row_order[p]];
             l = a.row_len[p];
-            vp = a.vop_ptr[p >> a.log2C];
+            if (l > kLongRow) l = 0;
+            start = a.rp[a.row_order[p]];
             rm = a.rm_ptr[p];
         }
-        wave_rows_foreach(l, [&](int i, int64_t j, bool act) {
-            const int64_t s = __shfl(start, i, 64), v0 = __shfl(vp, i, 64), r0 = __shfl(rm, i, 64);
-            if (act) {
-                const int64_t pp = p0 + i;
-                const int32_t c = a.pos_of[a.col[s + j]];
-                a.colidx[a.cstart[v0 + j] + (pp & (a.C - 1))] = c;
-                a.rm_col[r0 + j] = c;
-                if (a.push_cnt) atomicAdd(&a.push_cnt[c], 1u);
+        unsigned long long tot;
+        const unsigned long long ex = wave_exclusive_scan<unsigned long long>((unsigned long long)l, &tot);
+        for (unsigned long long e0 = 0; e0 < tot; e0 += 64 * U) {
+            int64_t src[U], dst[U];
+            bool act[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned long long e = e0 + (unsigned long long)(u * 64 + lane);
+                int lo = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const unsigned long long x = __shfl(ex, lo + step, 64);
+                    if (x <= e) lo += step;
+                }
+                const int64_t j = (int64_t)(e - __shfl(ex, lo, 64));
+                src[u] = __shfl(start, lo, 64) + j;
+                dst[u] = __shfl(rm, lo, 64) + j;
+                act[u] = e < tot;
             }
-        });
+            int32_t c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) c[u] = act[u] ? __builtin_nontemporal_load(&a.col[src[u]]) : 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) c[u] = act[u] ? a.pos_of[c[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!act[u]) continue;
+                __builtin_nontemporal_store(c[u], &a.rm_col[dst[u]]);
+                if (a.push_cnt) atomicAdd(&a.push_cnt[c[u]], 1u);
+            }
+        }
+    }
+}
+
+// positions of the rows longer than kLongRow
+__global__ __launch_bounds__(kBB) void k_collect_long(const int32_t *row_len, int32_t n, int32_t *list,
+                                                      unsigned int *count)
+{
+    for (int64_t p = (int64_t)blockIdx.x * kBB + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBB)
+        if (row_len[p] > kLongRow) list[atomicAdd(count, 1u)] = (int32_t)p;
+}
+
+// 6a'. the long rows: block (x, y) copies entries [x * kLongRow, +kLongRow)
+// of long row y, kLongRow / kBB independent entries per thread
+__global__ __launch_bounds__(kBB) void k_rowmajor_long(RowArgs a, const int32_t *list)
+{
+    const int32_t p = list[blockIdx.y];
+    const int32_t l = a.row_len[p];
+    const int64_t j0 = (int64_t)blockIdx.x * kLongRow;
+    if (j0 >= l) return;
+    const int64_t start = a.rp[a.row_order[p]], rm = a.rm_ptr[p];
+#pragma unroll 4
+    for (int k = 0; k < kLongRow / kBB; ++k) {
+        const int64_t j = j0 + k * kBB + threadIdx.x;
+        if (j < l) copy_entry(a, start + j, rm + j);
+    }
+}
+
+// 6b. the columns from the row-major copy.  A wave takes 64 consecutive
+// (slice, column) entries; their metadata are loaded once, one entry per
+// lane, and broadcast; the lanes are the slice's rows, so the stores of a
+// column are coalesced and each lane's loads walk its own row.
+__device__ __forceinline__ int64_t readlane64(int64_t v, int i)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, i);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), i);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__global__ __launch_bounds__(kBB) void k_colmajor(const int64_t *rm_ptr, const int32_t *rm_col,
+                                                  const int32_t *slice_of, const int64_t *vop_ptr,
+                                                  const uint8_t *vop_len, const int64_t *cstart, int64_t nvops,
+                                                  int32_t C, int32_t *colidx)
+{
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
+    const int nt = C > 64 ? 2 : 1; // lanes cover C rows
+    for (int64_t q0 = wave * 64; q0 < nvops; q0 += nwaves * 64) {
+        const int64_t q = q0 + lane;
+        int32_t sl = 0, act = 0;
+        int64_t j = 0, base = 0;
+        if (q < nvops) {
+            sl = slice_of[q];
+            j = q - vop_ptr[sl];
+            act = (int32_t)vop_len[q] + 1;
+            base = cstart[q];
+        }
+        const int cnt = (int)min((int64_t)64, nvops - q0);
+        int32_t cur = -1;
+        int64_t rmp0 = 0, rmp1 = 0;
+        for (int i = 0; i < cnt; ++i) {
+            const int32_t s_i = __builtin_amdgcn_readlane(sl, i);
+            if (s_i != cur) {
+                cur = s_i;
+                const int64_t r0 = (int64_t)s_i * C + lane;
+                rmp0 = lane < C ? rm_ptr[r0] : 0;
+                if (nt == 2) rmp1 = rm_ptr[r0 + 64];
+            }
+            const int64_t j_i = readlane64(j, i), b_i = readlane64(base, i);
+            const int32_t a_i = __builtin_amdgcn_readlane(act, i);
+            if (lane < a_i) colidx[b_i + lane] = rm_col[rmp0 + j_i];
+            if (nt == 2 && lane + 64 < a_i) colidx[b_i + 64 + lane] = rm_col[rmp1 + j_i];
+        }
     }
 }
 
@@ -274,13 +394,6 @@ __global__ __launch_bounds__(kBB) void k_push_fill(const int64_t *rm_ptr, const 
             }
         });
     }
-}
-
-int ilog2_exact(int x)
-{
-    int b = 0;
-    while ((1 << b) < x) ++b;
-    return b;
 }
 
 int bits_for(uint64_t x)
@@ -370,20 +483,26 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
 
     // 1.
     int32_t *len = nullptr;
-    unsigned int *flags = nullptr;
+    unsigned long long *flags = nullptr;
     if ((rc = T.get(n, &len)) || (rc = T.get(4, &flags))) return rc;
-    HIPCHK(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned int), s));
-    hipLaunchKernelGGL(k_lengths, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, rp, col, n, len, flags);
+    HIPCHK(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_entries, dim3(grid_for(nnz, kBB * 8)), dim3(kBB), 0, s, col, n, nnz, flags);
     HIPCHK(hipGetLastError());
-    unsigned int hflags[4];
+    unsigned long long hflags[4];
     HIPCHK(hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (hflags[2]) return albd_set_error_msg(ALBBA_EINVAL, "graph_build: column index outside [0, n)");
+    // every column index is checked before any of them is used as an index
+    if (hflags[1]) return albd_set_error_msg(ALBBA_EINVAL, "graph_build: column index outside [0, n)");
+    hipLaunchKernelGGL(k_rows, dim3(grid_for(n, kBB * 4)), dim3(kBB), 0, s, rp, col, n, nnz, len, flags);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     const int32_t maxlen = (int32_t)hflags[0];
-    rep->rows_sorted = hflags[1] ? 0 : 1;
+    const bool unsorted_rows = hflags[2] > hflags[3];
+    rep->rows_sorted = unsorted_rows ? 0 : 1;
 
     // 2.
-    if (hflags[1] && nnz > 0) {
+    if (unsorted_rows && nnz > 0) {
         int32_t *sorted = nullptr;
         if ((rc = T.get(nnz, &sorted))) return rc;
         size_t bytes = 0;
@@ -477,9 +596,11 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     g->vop_len = vop_len;
     if (!vop_len) return ALBBA_EDEVMEM;
     int64_t *csize = nullptr, *cstart = nullptr;
-    if ((rc = T.get(nvops + 1, &csize)) || (rc = T.get(nvops + 1, &cstart))) return rc;
+    int32_t *slice_of = nullptr;
+    if ((rc = T.get(nvops + 1, &csize)) || (rc = T.get(nvops + 1, &cstart)) || (rc = T.get(nvops + 1, &slice_of)))
+        return rc;
     hipLaunchKernelGGL(k_columns, dim3(grid_for(nvops + 1, kBB)), dim3(kBB), 0, s, row_len, vop_ptr, ns, C, pad,
-                       nvops, vop_len, csize);
+                       nvops, vop_len, csize, slice_of);
     HIPCHK(hipGetLastError());
     if ((rc = scan_i64(c, T, csize, cstart, (size_t)(nvops + 1)))) return rc;
     hipLaunchKernelGGL(k_slice_ptr, dim3(grid_for(ns_alloc + 1, kBB)), dim3(kBB), 0, s, vop_ptr, cstart, ns_alloc,
@@ -511,23 +632,41 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         if ((rc = T.get(npos + 1, &cnt))) return rc;
         HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(npos + 1), s));
     }
-    FillArgs fa;
-    fa.rp = rp;
-    fa.col = col;
-    fa.row_order = row_order;
-    fa.row_len = row_len;
-    fa.pos_of = pos_of;
-    fa.vop_ptr = vop_ptr;
-    fa.cstart = cstart;
-    fa.rm_ptr = rm_ptr;
-    fa.colidx = colidx;
-    fa.rm_col = rm_col;
-    fa.push_cnt = cnt;
-    fa.n = n;
-    fa.C = C;
-    fa.log2C = ilog2_exact(C);
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, fa);
+    RowArgs ra;
+    ra.rp = rp;
+    ra.col = col;
+    ra.row_order = row_order;
+    ra.row_len = row_len;
+    ra.pos_of = pos_of;
+    ra.rm_ptr = rm_ptr;
+    ra.rm_col = rm_col;
+    ra.push_cnt = cnt;
+    ra.n = n;
+    hipLaunchKernelGGL(k_rowmajor, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, ra);
     HIPCHK(hipGetLastError());
+    if (maxlen > kLongRow) {
+        int32_t *list = nullptr;
+        unsigned int *count = nullptr;
+        if ((rc = T.get(n, &list)) || (rc = T.get(1, &count))) return rc;
+        HIPCHK(hipMemsetAsync(count, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_collect_long, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, row_len, n, list, count);
+        HIPCHK(hipGetLastError());
+        unsigned int nlong = 0;
+        HIPCHK(hipMemcpyAsync(&nlong, count, sizeof nlong, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const unsigned int chunks = (unsigned int)((maxlen + kLongRow - 1) / kLongRow);
+        // grid y is limited to 65535: take the long rows in groups
+        for (unsigned int y0 = 0; y0 < nlong; y0 += 65535u) {
+            const unsigned int ny = nlong - y0 < 65535u ? nlong - y0 : 65535u;
+            hipLaunchKernelGGL(k_rowmajor_long, dim3(chunks, ny), dim3(kBB), 0, s, ra, list + y0);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    if (nvops > 0) {
+        hipLaunchKernelGGL(k_colmajor, dim3(grid_for(nvops, kBB)), dim3(kBB), 0, s, rm_ptr, rm_col, slice_of,
+                           vop_ptr, vop_len, cstart, nvops, C, colidx);
+        HIPCHK(hipGetLastError());
+    }
 
     // 7.
     if (in->symmetric != 1) {

@@ -95,6 +95,33 @@ def test_empty_rows_and_tiny_graphs(ctx):
             assert_same_layout(ctx, g, C_, sigma, pad).close()
 
 
+def star(n, directed):
+    """Vertex 0 joined to every other vertex (rows of W list in-neighbours:
+    directed, vertex 0's row is empty and every other row holds 0)."""
+    if directed:
+        rp = np.r_[0, np.arange(n)].astype(np.int64)
+        col = np.zeros(n - 1, np.int32)
+    else:
+        rp = np.r_[0, n - 1, np.arange(n, 2 * n - 1)].astype(np.int64)
+        col = np.r_[np.arange(1, n), np.zeros(n - 1)].astype(np.int32)
+    return rp, col
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_rows_longer_than_the_long_row_threshold(ctx, directed):
+    # the hub's row (or, directed, its push row) holds 9999 entries
+    rp, col = star(10000, directed)
+    g = A.Graph.from_arrays(rp, col, symmetric=not directed)
+    for C_, sigma, pad in [(64, 4096, 1), (128, 10000, 16), (16, 256, 32)]:
+        dg = assert_same_layout(ctx, g, C_, sigma, pad)
+        root = 0 if not directed else 5
+        o = dg.bfs(root)
+        lv, pa = O.bfs_queue(rp, col, root)
+        np.testing.assert_array_equal(o["level"], lv)
+        np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+
+
 def test_build_errors(ctx):
     g = A.Graph.rmat(8, 4, seed=105, perm_seed=106)
     for C_, sigma, pad in [(48, 4096, 1), (64, 100, 1), (64, 4096, 7)]:
