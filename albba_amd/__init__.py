@@ -108,6 +108,11 @@ class _BuildReport(C.Structure):
                 ("reserved_", C.c_int32)]
 
 
+class _LoadReport(C.Structure):
+    _fields_ = [("read_ms", C.c_float), ("prep_ms", C.c_float), ("build_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
 class _TuneConfig(C.Structure):
     _fields_ = [("C", C.c_int32), ("sigma", C.c_int32), ("mapping", C.c_int32),
                 ("status", C.c_int32), ("ms", C.c_double), ("batch_ms", C.c_double)]
@@ -150,6 +155,9 @@ def lib() -> C.CDLL:
         "albba_graph_build": (C.c_int, [C.c_void_p, C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
                                         C.POINTER(C.c_void_p), C.POINTER(_BuildReport)]),
         "albba_graph_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.POINTER(_SELL))]),
+        "albba_graph_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+        "albba_graph_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p), C.POINTER(_LoadReport)]),
         "albba_graph_free": (None, [C.c_void_p]),
         "albba_graph_device_bytes": (C.c_int64, [C.c_void_p]),
         "albba_bfs_opts_default": (None, [C.POINTER(_Opts)]),
@@ -425,6 +433,26 @@ class DeviceGraph:
         self.build_report = {"upload_ms": float(rep.upload_ms), "build_ms": float(rep.build_ms),
                              "rows_sorted": bool(rep.rows_sorted)}
         ctx._adopt(self)
+        return self
+
+    @classmethod
+    def load(cls, ctx: Context, path: str, C_: int = 64, sigma: int = 4096,
+             pad: int = 1) -> "DeviceGraph":
+        """Load a .cbd file into device memory and prepare and build it there
+        (albba_graph_load): the same device graph as
+        DeviceGraph(ctx, Graph.read(path).sellcs(C_, sigma, pad))."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self._p = C.c_void_p()
+        rep = _LoadReport()
+        _check(lib().albba_graph_load(ctx._p, path.encode(), C_, sigma, pad, C.byref(self._p),
+                                      C.byref(rep)))
+        self.build_report = {"read_ms": float(rep.read_ms), "prep_ms": float(rep.prep_ms),
+                             "build_ms": float(rep.build_ms), "total_ms": float(rep.total_ms)}
+        ctx._adopt(self)
+        n, nnz = C.c_int32(), C.c_int64()
+        _check(lib().albba_graph_size(self._p, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = int(n.value), int(nnz.value)
         return self
 
     def layout(self) -> "SellCS":

@@ -146,6 +146,13 @@ ALBD_HIDDEN int albd_read_counters(albd_ctx *c, const albd_state *st); /* sync *
 #define ALBD_NEVENTS 8
 ALBD_HIDDEN int albd_event_record(albd_ctx *c, int idx);
 ALBD_HIDDEN int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms); /* syncs b */
+ALBD_HIDDEN int albd_event_wait(albd_ctx *c, int idx); /* host waits for the event */
+
+/* ---- pinned host staging ---- */
+ALBD_HIDDEN int albd_pinned_alloc(albd_ctx *c, size_t bytes, void **p);
+ALBD_HIDDEN void albd_pinned_free(albd_ctx *c, void *p);
+/* host -> device copy on the context's stream, not waited for */
+ALBD_HIDDEN int albd_h2d_async(albd_ctx *c, void *dst, const void *src, size_t bytes);
 
 /* ---- fills (kernels, safe under graph capture) ---- */
 ALBD_HIDDEN int albd_fill_u32(albd_ctx *c, void *p, uint32_t v, size_t count);
@@ -155,13 +162,14 @@ ALBD_HIDDEN int albd_copy_u64(albd_ctx *c, void *dst, const void *src, size_t co
 
 /* ---- SELL-C-sigma build on the device ---- */
 typedef struct albd_build_input {
-    const int64_t *row_ptr; /* host CSR (validated by the caller: row_ptr[0] = 0,
-                               non-decreasing, row_ptr[n] = nnz)              */
+    const int64_t *row_ptr; /* CSR (validated by the caller: row_ptr[0] = 0,
+                               non-decreasing, row_ptr[n] = nnz), host memory
+                               unless on_device                               */
     const int32_t *col;
     int64_t nnz;
     int32_t n, C, sigma, pad;
     int32_t symmetric;      /* 1: the push CSR is the layout's own rows      */
-    int32_t pad_;
+    int32_t on_device;      /* 1: row_ptr/col are device pointers (no upload) */
 } albd_build_input;
 typedef struct albd_build_report {
     float upload_ms;        /* CSR host -> device                            */
@@ -175,6 +183,14 @@ typedef void *(*albd_build_alloc)(void *arg, size_t bytes);
  * push and pull CSRs) from the CSR; synchronous.  Temporaries are freed. */
 ALBD_HIDDEN int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_graph *g,
                                  albd_build_alloc alloc, void *alloc_arg, albd_build_report *rep);
+
+/* W = A^T of a device CSR with int32 row pointers (the reference's working
+ * matrix, bfs/src/main.c:117-126): checks row_ptr (0 .. nnz, non-decreasing)
+ * and the column range (*bad = 1 and no output when either fails), then
+ * allocates *wp (int64, n+1) and *wc (nnz) -- freed by the caller with
+ * albd_free -- with every row ascending.  Synchronous. */
+ALBD_HIDDEN int albd_transpose_csr(albd_ctx *c, const int32_t *rp32, const int32_t *col, int32_t n,
+                                   int64_t nnz, int64_t **wp, int32_t **wc, int *bad);
 
 /* ---- BFS pieces ---- */
 /* haslen[w] from row_len */

@@ -194,6 +194,39 @@ int albd_event_record(albd_ctx *c, int idx)
     return 0;
 }
 
+int albd_event_wait(albd_ctx *c, int idx)
+{
+    HIPCHK(hipEventSynchronize(c->ev[idx]));
+    return 0;
+}
+
+int albd_pinned_alloc(albd_ctx *c, size_t bytes, void **p)
+{
+    HIPCHK(hipSetDevice(c->device));
+    hipError_t e = hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        snprintf(g_err, sizeof g_err, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return ALBBA_ENOMEM;
+    }
+    return 0;
+}
+
+void albd_pinned_free(albd_ctx *c, void *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(c->device);
+    (void)hipHostFree(p);
+}
+
+int albd_h2d_async(albd_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
 int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms)
 {
     HIPCHK(hipEventSynchronize(c->ev[b]));

@@ -396,6 +396,28 @@ __global__ __launch_bounds__(kBB) void k_push_fill(const int64_t *rm_ptr, const 
     }
 }
 
+// int32 row pointers widened to int64; flags[0] set unless they run from 0
+// to nnz without decreasing
+__global__ __launch_bounds__(kBB) void k_widen_rp(const int32_t *rp32, int32_t n, int64_t nnz, int64_t *rp64,
+                                                  unsigned long long *flags)
+{
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBB + threadIdx.x; i <= n; i += (int64_t)gridDim.x * kBB) {
+        const int32_t v = rp32[i];
+        rp64[i] = v;
+        if (i == 0 && v != 0) bad = true;
+        if (i == n && (int64_t)v != nnz) bad = true;
+        if (i < n && rp32[i + 1] < v) bad = true;
+    }
+    if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(&flags[0], 1ull);
+}
+
+__global__ __launch_bounds__(kBB) void k_hist(const int32_t *col, int64_t nnz, unsigned int *cnt)
+{
+    for (int64_t k = (int64_t)blockIdx.x * kBB + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBB)
+        atomicAdd(&cnt[col[k]], 1u);
+}
+
 int bits_for(uint64_t x)
 {
     int b = 0;
@@ -464,9 +486,18 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     int rc;
 
     // the CSR on the device
-    int64_t *rp = nullptr;
-    int32_t *col = nullptr;
-    if ((rc = T.get(n + 1, &rp)) || (rc = T.get(nnz > 0 ? nnz : 1, &col))) return rc;
+    const int64_t *rp = nullptr;
+    const int32_t *col = nullptr;
+    int64_t *rp_up = nullptr;
+    int32_t *col_up = nullptr;
+    if (in->on_device) {
+        rp = in->row_ptr;
+        col = in->col;
+    } else {
+        if ((rc = T.get(n + 1, &rp_up)) || (rc = T.get(nnz > 0 ? nnz : 1, &col_up))) return rc;
+        rp = rp_up;
+        col = col_up;
+    }
     hipEvent_t ev[4];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     struct EvFree {
@@ -477,8 +508,11 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         }
     } evfree{ev};
     HIPCHK(hipEventRecord(ev[0], s));
-    HIPCHK(hipMemcpyAsync(rp, in->row_ptr, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, s));
-    if (nnz > 0) HIPCHK(hipMemcpyAsync(col, in->col, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    if (!in->on_device) {
+        HIPCHK(hipMemcpyAsync(rp_up, in->row_ptr, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, s));
+        if (nnz > 0)
+            HIPCHK(hipMemcpyAsync(col_up, in->col, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    }
     HIPCHK(hipEventRecord(ev[1], s));
 
     // 1.
@@ -701,5 +735,88 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     HIPCHK(hipEventElapsedTime(&bu, ev[1], ev[2]));
     rep->upload_ms = up;
     rep->build_ms = bu;
+    return 0;
+}
+
+extern "C" int albd_transpose_csr(albd_ctx *c, const int32_t *rp32, const int32_t *col, int32_t n, int64_t nnz,
+                                  int64_t **wp_out, int32_t **wc_out, int *bad)
+{
+    *wp_out = nullptr;
+    *wc_out = nullptr;
+    *bad = 0;
+    hipStream_t s = albd_stream(c);
+    HIPCHK(hipSetDevice(albd_ctx_device(c)));
+    Temps T(c);
+    int rc;
+    int64_t *rp64 = nullptr;
+    unsigned long long *flags = nullptr;
+    if ((rc = T.get(n + 1, &rp64)) || (rc = T.get(4, &flags))) return rc;
+    HIPCHK(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_widen_rp, dim3(grid_for((int64_t)n + 1, kBB)), dim3(kBB), 0, s, rp32, n, nnz, rp64, flags);
+    HIPCHK(hipGetLastError());
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_entries, dim3(grid_for(nnz, kBB * 8)), dim3(kBB), 0, s, col, n, nnz, flags);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long hflags[4];
+    HIPCHK(hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hflags[0] || hflags[1]) {
+        *bad = 1;
+        return 0;
+    }
+    // W's row lengths are A's column counts
+    unsigned int *cnt = nullptr;
+    int64_t *len64 = nullptr;
+    if ((rc = T.get(n + 1, &cnt)) || (rc = T.get(n + 1, &len64))) return rc;
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(n + 1), s));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_hist, dim3(grid_for(nnz, kBB * 8)), dim3(kBB), 0, s, col, nnz, cnt);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_widen_u32, dim3(grid_for((int64_t)n + 1, kBB)), dim3(kBB), 0, s, cnt, (int64_t)n + 1,
+                       len64);
+    HIPCHK(hipGetLastError());
+    int64_t *wp = nullptr;
+    int32_t *wc = nullptr;
+    if ((rc = albd_malloc(c, sizeof(int64_t) * (size_t)(n + 1), (void **)&wp))) return rc;
+    if ((rc = albd_malloc(c, sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1), (void **)&wc))) {
+        albd_free(c, wp);
+        return rc;
+    }
+    struct Out {
+        albd_ctx *c;
+        int64_t *wp;
+        int32_t *wc;
+        bool keep = false;
+        ~Out()
+        {
+            if (!keep) {
+                albd_free(c, wp);
+                albd_free(c, wc);
+            }
+        }
+    } out{c, wp, wc};
+    if ((rc = scan_i64(c, T, len64, wp, (size_t)n + 1))) return rc;
+    if (nnz > 0) {
+        // each entry (r, c) of A lands in W's row c, in any order; the rows
+        // are then sorted, which lists r ascending as a row-by-row scatter would
+        int32_t *unsorted = nullptr;
+        if ((rc = T.get(nnz, &unsorted))) return rc;
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(n + 1), s));
+        hipLaunchKernelGGL(k_push_fill, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, rp64, col, n, wp, cnt, unsorted);
+        HIPCHK(hipGetLastError());
+        size_t bytes = 0;
+        ROCCHK(rocprim::segmented_radix_sort_keys(nullptr, bytes, unsorted, wc, (unsigned int)nnz, (unsigned int)n,
+                                                  wp, wp + 1, 0, 32, s));
+        unsigned char *tmp = nullptr;
+        if ((rc = T.get(bytes ? bytes : 1, &tmp))) return rc;
+        ROCCHK(rocprim::segmented_radix_sort_keys(tmp, bytes, unsorted, wc, (unsigned int)nnz, (unsigned int)n, wp,
+                                                  wp + 1, 0, 32, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    out.keep = true;
+    *wp_out = wp;
+    *wc_out = wc;
     return 0;
 }

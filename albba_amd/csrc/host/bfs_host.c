@@ -367,6 +367,130 @@ int albba_graph_build(albba_ctx *ctx, const albba_csr *cg, int C, int sigma, int
     return ALBBA_OK;
 }
 
+/* Stream `bytes` of f (from its current offset) into device memory at dst
+ * through two pinned buffers: reading one while the other's copy runs. */
+#define LOAD_CHUNK ((size_t)64 << 20)
+static int stream_to_device(albd_ctx *d, FILE *f, void *dst, size_t bytes, void *buf[2])
+{
+    size_t done = 0;
+    int k = 0, rc;
+    while (done < bytes) {
+        const size_t len = bytes - done < LOAD_CHUNK ? bytes - done : LOAD_CHUNK;
+        /* the copy that last used this buffer must have finished */
+        if ((rc = albd_event_wait(d, 4 + k))) return rc;
+        if (fread(buf[k], 1, len, f) != len) return albba_fail(ALBBA_EIO, "graph_load: short read");
+        if ((rc = albd_h2d_async(d, (char *)dst + done, buf[k], len)) || (rc = albd_event_record(d, 4 + k)))
+            return rc;
+        done += len;
+        k ^= 1;
+    }
+    return 0;
+}
+
+int albba_graph_load(albba_ctx *ctx, const char *path, int C, int sigma, int pad, albba_graph **out,
+                     albba_load_report *report)
+{
+    if (!ctx || !path || !out) return albba_fail(ALBBA_EINVAL, "graph_load: null argument");
+    if (C != 16 && C != 32 && C != 64 && C != 128)
+        return albba_fail(ALBBA_EINVAL, "graph_load: C must be 16, 32, 64 or 128 (got %d)", C);
+    if (pad != 1 && pad != 16 && pad != 32)
+        return albba_fail(ALBBA_EINVAL, "graph_load: pad must be 1, 16 or 32 (got %d)", pad);
+    const size_t L = strlen(path);
+    if (L < 4 || strcmp(path + L - 4, ".cbd") != 0)
+        return albba_fail(ALBBA_EINVAL, "graph_load: %s is not a .cbd file (load .mtx with albba_read_graph)", path);
+    const double t0 = albba_now_ms();
+    FILE *f = fopen(path, "rb");
+    if (!f) return albba_fail(ALBBA_EIO, "graph_load: cannot open %s", path);
+    /* bfs/src/biio.h:21-71: int32 nrows, ncols, nnz; int32 row_ptr[nrows+1];
+     * int32 col[nnz]; double val[nnz] (not read: the BFS ignores values) */
+    int32_t hdr[3];
+    if (fread(hdr, sizeof(int32_t), 3, f) != 3) { fclose(f); return albba_fail(ALBBA_EIO, "graph_load: short header"); }
+    const int32_t n = hdr[0];
+    const int64_t nnz = hdr[2];
+    if (hdr[0] != hdr[1] || n < 1 || nnz < 0) {
+        fclose(f);
+        return albba_fail(ALBBA_EINVAL, "graph_load: %d x %d matrix with %lld entries", hdr[0], hdr[1], (long long)nnz);
+    }
+    if (sigma < 1 || (sigma % C != 0 && sigma < n)) {
+        fclose(f);
+        return albba_fail(ALBBA_EINVAL, "graph_load: sigma (%d) must be a multiple of C (%d) or >= n", sigma, C);
+    }
+    albd_ctx *d = ctx->d;
+    int32_t *rp32 = NULL, *col = NULL;
+    void *buf[2] = {NULL, NULL};
+    int rc = albd_malloc(d, sizeof(int32_t) * ((size_t)n + 1), (void **)&rp32);
+    if (!rc) rc = albd_malloc(d, sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1), (void **)&col);
+    if (!rc) rc = albd_pinned_alloc(d, LOAD_CHUNK, &buf[0]);
+    if (!rc) rc = albd_pinned_alloc(d, LOAD_CHUNK, &buf[1]);
+    /* a fresh pair of events: waiting on one never recorded returns at once */
+    if (!rc) rc = albd_event_record(d, 4);
+    if (!rc) rc = albd_event_record(d, 5);
+    if (!rc) rc = stream_to_device(d, f, rp32, sizeof(int32_t) * ((size_t)n + 1), buf);
+    if (!rc) rc = stream_to_device(d, f, col, sizeof(int32_t) * (size_t)nnz, buf);
+    if (!rc) rc = albd_sync(d);
+    fclose(f);
+    albd_pinned_free(d, buf[0]);
+    albd_pinned_free(d, buf[1]);
+    const double t1 = albba_now_ms();
+    int64_t *wp = NULL;
+    int32_t *wc = NULL;
+    int bad = 0;
+    if (!rc) rc = albd_transpose_csr(d, rp32, col, n, nnz, &wp, &wc, &bad);
+    albd_free(d, rp32);
+    albd_free(d, col);
+    if (rc || bad) {
+        char buf2[512];
+        snprintf(buf2, sizeof buf2, "%s", bad ? "row pointers or column indices out of range" :
+                 (rc == ALBBA_EIO ? albba_last_error() : albd_error()));
+        albd_free(d, wp);
+        albd_free(d, wc);
+        return albba_fail(bad ? ALBBA_EINVAL : rc, "graph_load: %s", buf2);
+    }
+    const double t2 = albba_now_ms();
+    albba_graph *G = (albba_graph *)albba_calloc(1, sizeof(albba_graph));
+    if (!G) { albd_free(d, wp); albd_free(d, wc); return albba_fail(ALBBA_ENOMEM, "graph_load"); }
+    G->ctx = ctx;
+    G->n = n;
+    G->nnz = nnz;
+    G->sigma = sigma;
+    albd_build_input in;
+    memset(&in, 0, sizeof in);
+    in.row_ptr = wp;
+    in.col = wc;
+    in.nnz = nnz;
+    in.n = n;
+    in.C = C;
+    in.sigma = sigma;
+    in.pad = pad;
+    in.symmetric = 0; /* the .cbd path leaves isSymmetric unset: W is built as directed */
+    in.on_device = 1;
+    albd_build_report r;
+    memset(&r, 0, sizeof r);
+    rc = albd_build_graph(d, &in, &G->g, build_alloc, G, &r);
+    albd_free(d, wp);
+    albd_free(d, wc);
+    if (rc) {
+        char buf2[512];
+        snprintf(buf2, sizeof buf2, "%s", albd_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_load: %s", buf2);
+    }
+    if ((rc = graph_state_alloc(G))) {
+        char buf2[512];
+        snprintf(buf2, sizeof buf2, "%s", albba_last_error());
+        albba_graph_free(G);
+        return albba_fail(rc, "graph_load: %s", buf2);
+    }
+    if (report) {
+        report->read_ms = (float)(t1 - t0);
+        report->prep_ms = (float)(t2 - t1);
+        report->build_ms = r.build_ms;
+        report->total_ms = (float)(albba_now_ms() - t0);
+    }
+    *out = G;
+    return ALBBA_OK;
+}
+
 int albba_graph_layout(albba_ctx *ctx, const albba_graph *G, albba_sellcs **out)
 {
     if (!ctx || !G || !out || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "graph_layout: bad argument");
@@ -427,6 +551,14 @@ int albba_graph_layout(albba_ctx *ctx, const albba_graph *G, albba_sellcs **out)
 }
 
 int64_t albba_graph_device_bytes(const albba_graph *G) { return G ? G->device_bytes : 0; }
+
+int albba_graph_size(const albba_graph *G, int32_t *n, int64_t *nnz)
+{
+    if (!G || !n || !nnz) return albba_fail(ALBBA_EINVAL, "graph_size: null argument");
+    *n = G->n;
+    *nnz = G->nnz;
+    return ALBBA_OK;
+}
 
 /* ---- level loop ---- */
 

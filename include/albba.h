@@ -304,11 +304,29 @@ typedef struct albba_build_report {
 } albba_build_report;
 ALBBA_API int albba_graph_build(albba_ctx *ctx, const albba_csr *g, int C, int sigma, int pad,
                                 albba_graph **out, albba_build_report *report);
+/* Load a .cbd file (bfs/src/biio.h:21-71 layout) straight into device
+ * memory -- the row pointers and columns stream through pinned buffers, file
+ * reads overlapping the copies; the values are not read -- then form the
+ * reference's working matrix W = A^T with rows ascending on the device
+ * (bfs/src/main.c:117-126, the DIRECTED build; the .cbd path sets no symmetry
+ * flag) and build the layout there (as albba_graph_build).  The same device
+ * graph as albba_read_graph + albba_sellcs_build + albba_graph_upload.
+ * .mtx files are text: load them with albba_read_graph.  report may be NULL. */
+typedef struct albba_load_report {
+    float read_ms;   /* file -> device, overlapped                          */
+    float prep_ms;   /* checks and the transpose on the device              */
+    float build_ms;  /* the layout on the device                            */
+    float total_ms;  /* the whole call                                      */
+} albba_load_report;
+ALBBA_API int albba_graph_load(albba_ctx *ctx, const char *path, int C, int sigma, int pad,
+                               albba_graph **out, albba_load_report *report);
 /* Copy a device graph's layout to the host as an albba_sellcs (free with
  * albba_sellcs_free); row_lengths and push_ptr cover nslices * C positions. */
 ALBBA_API int albba_graph_layout(albba_ctx *ctx, const albba_graph *g, albba_sellcs **out);
 ALBBA_API void albba_graph_free(albba_graph *g);
 ALBBA_API int64_t albba_graph_device_bytes(const albba_graph *g);
+/* Vertices and stored entries of a device graph. */
+ALBBA_API int albba_graph_size(const albba_graph *g, int32_t *n, int64_t *nnz);
 
 /* Device copy bandwidth: copies `bytes` (rounded down to 16) between two
  * fresh device buffers `reps` times on the context's stream and returns the

@@ -136,3 +136,77 @@ def test_build_errors(ctx):
     if rp[3] != rp[4]:
         with pytest.raises(A.AlbbaError, match="row_ptr"):
             A.DeviceGraph.build(ctx, A.Graph.from_arrays(rp, g.col, symmetric=True), 64, 4096, 1)
+
+
+def assert_same_as_host_path(ctx, path, C_, sigma, pad):
+    host = A.Graph.read(str(path))
+    ref = host.sellcs(C_, sigma, pad).arrays()
+    dg = A.DeviceGraph.load(ctx, str(path), C_, sigma, pad)
+    assert (dg.n, dg.nnz) == (host.n, host.nnz)
+    got = dg.layout().arrays()
+    for k, v in ref.items():
+        if v is None:
+            assert got[k] is None, k
+        else:
+            np.testing.assert_array_equal(got[k], v, err_msg=f"{k} C={C_} sigma={sigma} pad={pad}")
+    return dg, host
+
+
+@pytest.mark.parametrize("kind", ["rmat", "directed", "golden_dir"])
+def test_cbd_load_equals_host_read_build_upload(ctx, tmp_path, kind):
+    """.cbd straight into device memory, W = A^T formed and the layout built
+    on the device: the same arrays as the host reader + builder + upload."""
+    if kind == "rmat":
+        g = A.Graph.rmat(12, 16, seed=111, perm_seed=112)
+    elif kind == "directed":
+        base = A.Graph.er(5000, 8.0, seed=113, perm_seed=114)
+        rows = np.repeat(np.arange(base.n), np.diff(base.row_ptr))
+        keep = (base.col * 7 + rows) % 3 == 0  # an arbitrary asymmetric subset
+        rp = np.zeros(base.n + 1, np.int64)
+        np.cumsum(np.bincount(rows[keep], minlength=base.n), out=rp[1:])
+        g = A.Graph.from_arrays(rp, base.col[keep].astype(np.int32), symmetric=False)
+    else:
+        name = [k for k in GOLDEN if k.startswith("dir_")][0]
+        g = A.Graph.from_arrays(GG[f"{name}__rp"], GG[f"{name}__col"], symmetric=False)
+    path = tmp_path / "g.cbd"
+    g.write_cbd(str(path))
+    for C_, sigma, pad in [(64, 4096, 1), (32, 256, 16)]:
+        dg, host = assert_same_as_host_path(ctx, path, C_, sigma, pad)
+        for root in host.pick_roots(2, 115):
+            o = dg.bfs(int(root))
+            lv, pa = O.bfs_queue(host.row_ptr, host.col, int(root))
+            np.testing.assert_array_equal(o["level"], lv)
+            np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+
+
+def test_cbd_load_errors(ctx, tmp_path):
+    g = A.Graph.rmat(9, 8, seed=116, perm_seed=117)
+    good = tmp_path / "g.cbd"
+    g.write_cbd(str(good))
+    raw = good.read_bytes()
+    n, nnz = g.n, g.nnz
+    # cut inside the column array
+    cut = tmp_path / "cut.cbd"
+    cut.write_bytes(raw[:12 + 4 * (n + 1) + 4 * (nnz // 2)])
+    with pytest.raises(A.AlbbaError, match="short read"):
+        A.DeviceGraph.load(ctx, str(cut))
+    # a column index outside [0, n)
+    bad = bytearray(raw)
+    off = 12 + 4 * (n + 1) + 4 * 3
+    bad[off:off + 4] = np.int32(n + 5).tobytes()
+    badp = tmp_path / "bad.cbd"
+    badp.write_bytes(bytes(bad))
+    with pytest.raises(A.AlbbaError, match="out of range"):
+        A.DeviceGraph.load(ctx, str(badp))
+    # not square
+    ns = bytearray(raw)
+    ns[4:8] = np.int32(n + 1).tobytes()
+    nsp = tmp_path / "ns.cbd"
+    nsp.write_bytes(bytes(ns))
+    with pytest.raises(A.AlbbaError):
+        A.DeviceGraph.load(ctx, str(nsp))
+    with pytest.raises(A.AlbbaError, match="not a .cbd"):
+        A.DeviceGraph.load(ctx, str(tmp_path / "g.mtx"))
+    # the good file still loads on the same context afterwards
+    A.DeviceGraph.load(ctx, str(good)).close()
