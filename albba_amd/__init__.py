@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libalbba.so")
 CHOICE_PUSH, CHOICE_PULL, CHOICE_PULL_SB, CHOICE_PULL_CB = 0, 1, 2, 3
 CHOICE_NAMES = ("push", "pull", "pull_sb", "pull_cb")
 SWITCH_HEURISTIC, SWITCH_LEARNED, SWITCH_FIXED, SWITCH_SCRIPT, SWITCH_BEST = 0, 1, 2, 3, 4
+GEN_RMAT, GEN_ER, GEN_GRID = 0, 1, 2
 NUM_FEATURES = 12
 MLP_NPARAM = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 4 + 4
 ERRORS = {
@@ -108,6 +109,18 @@ class _BuildReport(C.Structure):
                 ("reserved_", C.c_int32)]
 
 
+class _GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("scale", C.c_int32), ("edgefactor", C.c_int32),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("n", C.c_int32),
+                ("rows", C.c_int32), ("cols", C.c_int32), ("mean_degree", C.c_double),
+                ("p_remove", C.c_double), ("seed", C.c_uint64), ("perm_seed", C.c_uint64)]
+
+
+class _GenReport(C.Structure):
+    _fields_ = [("perm_ms", C.c_float), ("gen_ms", C.c_float), ("build_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
 class _LoadReport(C.Structure):
     _fields_ = [("read_ms", C.c_float), ("prep_ms", C.c_float), ("build_ms", C.c_float),
                 ("total_ms", C.c_float)]
@@ -155,6 +168,9 @@ def lib() -> C.CDLL:
         "albba_graph_build": (C.c_int, [C.c_void_p, C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
                                         C.POINTER(C.c_void_p), C.POINTER(_BuildReport)]),
         "albba_graph_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.POINTER(_SELL))]),
+        "albba_graph_generate": (C.c_int, [C.c_void_p, C.POINTER(_GenParams), C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_void_p), C.POINTER(_CSR),
+                                           C.POINTER(_GenReport)]),
         "albba_graph_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
         "albba_graph_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(C.c_void_p), C.POINTER(_LoadReport)]),
@@ -454,6 +470,42 @@ class DeviceGraph:
         _check(lib().albba_graph_size(self._p, C.byref(n), C.byref(nnz)))
         self.n, self.nnz = int(n.value), int(nnz.value)
         return self
+
+    @classmethod
+    def generate(cls, ctx: Context, kind: str, C_: int = 64, sigma: int = 4096, pad: int = 1,
+                 host_copy: bool = True, **kw):
+        """Generate on the device (albba_graph_generate) the graph that
+        Graph.rmat / Graph.er / Graph.grid would return, and build its layout
+        there.  kind: "rmat" (scale, edgefactor=16, a=.57, b=.19, c=.19),
+        "er" (n, mean_degree) or "grid" (rows, cols, p_remove=0.1); seed,
+        perm_seed.  Returns (DeviceGraph, Graph or None)."""
+        p = _GenParams()
+        p.seed, p.perm_seed = kw.get("seed", 1), kw.get("perm_seed", 2)
+        if kind == "rmat":
+            p.kind, p.scale, p.edgefactor = GEN_RMAT, kw["scale"], kw.get("edgefactor", 16)
+            p.a, p.b, p.c = kw.get("a", 0.57), kw.get("b", 0.19), kw.get("c", 0.19)
+        elif kind == "er":
+            p.kind, p.n, p.mean_degree = GEN_ER, kw["n"], kw["mean_degree"]
+        elif kind == "grid":
+            p.kind, p.rows, p.cols = GEN_GRID, kw["rows"], kw["cols"]
+            p.p_remove = kw.get("p_remove", 0.1)
+            p.seed = kw.get("seed", 4)
+        else:
+            raise ValueError(f"unknown kind {kind!r}")
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self._p = C.c_void_p()
+        raw = _CSR()
+        rep = _GenReport()
+        _check(lib().albba_graph_generate(ctx._p, C.byref(p), C_, sigma, pad, C.byref(self._p),
+                                          C.byref(raw) if host_copy else None, C.byref(rep)))
+        self.build_report = {"perm_ms": float(rep.perm_ms), "gen_ms": float(rep.gen_ms),
+                             "build_ms": float(rep.build_ms), "total_ms": float(rep.total_ms)}
+        ctx._adopt(self)
+        n, nnz = C.c_int32(), C.c_int64()
+        _check(lib().albba_graph_size(self._p, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = int(n.value), int(nnz.value)
+        return self, (Graph(raw, owned=True) if host_copy else None)
 
     def layout(self) -> "SellCS":
         """The device layout copied to the host (albba_graph_layout)."""

@@ -192,6 +192,24 @@ ALBD_HIDDEN int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_g
 ALBD_HIDDEN int albd_transpose_csr(albd_ctx *c, const int32_t *rp32, const int32_t *col, int32_t n,
                                    int64_t nnz, int64_t **wp, int32_t **wc, int *bad);
 
+/* Generated graph on the device (dev/gen.hip): the edge rule of `kind`
+ * (ALBBA_GEN_*), ids mapped through perm (host array, n entries), both
+ * directions, self loops and duplicates dropped, rows ascending.  Allocates
+ * *rp (int64, n+1) and *col (m) -- freed by the caller with albd_free.
+ * Synchronous. */
+typedef struct albd_gen_spec {
+    int32_t kind, scale;
+    int32_t n, bits;        /* bits: id field width of the sort keys      */
+    int64_t ne;             /* edges drawn                                 */
+    double a, ab, abc;      /* RMAT thresholds                             */
+    uint64_t seed;
+    int32_t rows, cols;
+    int64_t H;              /* grid: horizontal edges                      */
+    double p_remove;
+} albd_gen_spec;
+ALBD_HIDDEN int albd_gen_csr(albd_ctx *c, const albd_gen_spec *spec, const int32_t *perm_host,
+                             int64_t **rp, int32_t **col, int64_t *m);
+
 /* ---- BFS pieces ---- */
 /* haslen[w] from row_len */
 ALBD_HIDDEN int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen);

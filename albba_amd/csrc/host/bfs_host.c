@@ -491,6 +491,139 @@ int albba_graph_load(albba_ctx *ctx, const char *path, int C, int sigma, int pad
     return ALBBA_OK;
 }
 
+static int bits_for_ids(int64_t n)
+{
+    int b = 1;
+    while (b < 62 && ((int64_t)1 << b) < n) ++b;
+    return b;
+}
+
+int albba_graph_generate(albba_ctx *ctx, const albba_gen_params *p, int C, int sigma, int pad,
+                         albba_graph **out, albba_csr *host_copy, albba_gen_report *report)
+{
+    if (!ctx || !p || !out) return albba_fail(ALBBA_EINVAL, "graph_generate: null argument");
+    if (C != 16 && C != 32 && C != 64 && C != 128)
+        return albba_fail(ALBBA_EINVAL, "graph_generate: C must be 16, 32, 64 or 128 (got %d)", C);
+    if (pad != 1 && pad != 16 && pad != 32)
+        return albba_fail(ALBBA_EINVAL, "graph_generate: pad must be 1, 16 or 32 (got %d)", pad);
+    /* the host generators' parameter rules (host/graphgen.c) */
+    albd_gen_spec g;
+    memset(&g, 0, sizeof g);
+    g.kind = p->kind;
+    g.seed = p->seed;
+    if (p->kind == ALBBA_GEN_RMAT) {
+        if (p->scale < 1 || p->scale > 30 || p->edgefactor < 1 || p->a < 0 || p->b < 0 || p->c < 0 ||
+            p->a + p->b + p->c > 1.0 + 1e-12)
+            return albba_fail(ALBBA_EINVAL, "graph_generate: bad RMAT parameters");
+        g.scale = p->scale;
+        g.n = (int32_t)1 << p->scale;
+        g.ne = (int64_t)p->edgefactor << p->scale;
+        g.a = p->a;
+        g.ab = p->a + p->b;
+        g.abc = p->a + p->b + p->c;
+        g.bits = p->scale;
+    } else if (p->kind == ALBBA_GEN_ER) {
+        if (p->n < 1 || p->mean_degree < 0) return albba_fail(ALBBA_EINVAL, "graph_generate: bad ER parameters");
+        g.n = p->n;
+        g.ne = (int64_t)((double)p->n * p->mean_degree / 2.0 + 0.5);
+        g.bits = bits_for_ids(g.n);
+    } else if (p->kind == ALBBA_GEN_GRID) {
+        if (p->rows < 1 || p->cols < 1 || (int64_t)p->rows * p->cols > INT32_MAX || p->p_remove < 0 ||
+            p->p_remove > 1)
+            return albba_fail(ALBBA_EINVAL, "graph_generate: bad grid parameters");
+        g.rows = p->rows;
+        g.cols = p->cols;
+        g.n = p->rows * p->cols;
+        g.H = (int64_t)p->rows * (p->cols - 1);
+        g.ne = g.H + (int64_t)(p->rows - 1) * p->cols;
+        g.p_remove = p->p_remove;
+        g.bits = bits_for_ids(g.n);
+    } else {
+        return albba_fail(ALBBA_EINVAL, "graph_generate: unknown kind %d", p->kind);
+    }
+    if (2 * g.ne > INT32_MAX) return albba_fail(ALBBA_ERANGE, "graph_generate: nnz beyond int32");
+    if (sigma < 1 || (sigma % C != 0 && sigma < g.n))
+        return albba_fail(ALBBA_EINVAL, "graph_generate: sigma (%d) must be a multiple of C (%d) or >= n", sigma, C);
+    const double t0 = albba_now_ms();
+    int32_t *perm = albba_make_perm(g.n, p->perm_seed);
+    if (!perm) return albba_fail(ALBBA_ENOMEM, "graph_generate: permutation");
+    const double t1 = albba_now_ms();
+    albd_ctx *d = ctx->d;
+    int64_t *rp = NULL, m = 0;
+    int32_t *col = NULL;
+    int rc = albd_gen_csr(d, &g, perm, &rp, &col, &m);
+    free(perm);
+    if (rc) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", albd_error());
+        return albba_fail(rc, "graph_generate: %s", buf);
+    }
+    const double t2 = albba_now_ms();
+    if (host_copy) {
+        memset(host_copy, 0, sizeof *host_copy);
+        int64_t *hrp = (int64_t *)albba_malloc(sizeof(int64_t) * ((size_t)g.n + 1));
+        int32_t *hcol = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+        if (!hrp || !hcol) {
+            free(hrp); free(hcol); albd_free(d, rp); albd_free(d, col);
+            return albba_fail(ALBBA_ENOMEM, "graph_generate: host copy");
+        }
+        if ((rc = albd_d2h(d, hrp, rp, sizeof(int64_t) * ((size_t)g.n + 1))) ||
+            (rc = albd_d2h(d, hcol, col, sizeof(int32_t) * (size_t)m))) {
+            char buf[512];
+            snprintf(buf, sizeof buf, "%s", albd_error());
+            free(hrp); free(hcol); albd_free(d, rp); albd_free(d, col);
+            return albba_fail(rc, "graph_generate: %s", buf);
+        }
+        host_copy->n = g.n;
+        host_copy->nnz = m;
+        host_copy->row_ptr = hrp;
+        host_copy->col = hcol;
+        host_copy->symmetric = 1;
+    }
+    albba_graph *G = (albba_graph *)albba_calloc(1, sizeof(albba_graph));
+    if (!G) {
+        albd_free(d, rp); albd_free(d, col);
+        if (host_copy) albba_csr_free(host_copy);
+        return albba_fail(ALBBA_ENOMEM, "graph_generate");
+    }
+    G->ctx = ctx;
+    G->n = g.n;
+    G->nnz = m;
+    G->sigma = sigma;
+    albd_build_input in;
+    memset(&in, 0, sizeof in);
+    in.row_ptr = rp;
+    in.col = col;
+    in.nnz = m;
+    in.n = g.n;
+    in.C = C;
+    in.sigma = sigma;
+    in.pad = pad;
+    in.symmetric = 1;
+    in.on_device = 1;
+    albd_build_report r;
+    memset(&r, 0, sizeof r);
+    rc = albd_build_graph(d, &in, &G->g, build_alloc, G, &r);
+    albd_free(d, rp);
+    albd_free(d, col);
+    char buf[512] = "";
+    if (rc) snprintf(buf, sizeof buf, "%s", albd_error());
+    else if ((rc = graph_state_alloc(G))) snprintf(buf, sizeof buf, "%s", albba_last_error());
+    if (!rc) {
+        if (report) {
+            report->perm_ms = (float)(t1 - t0);
+            report->gen_ms = (float)(t2 - t1);
+            report->build_ms = r.build_ms;
+            report->total_ms = (float)(albba_now_ms() - t0);
+        }
+        *out = G;
+        return ALBBA_OK;
+    }
+    albba_graph_free(G);
+    if (host_copy) albba_csr_free(host_copy);
+    return albba_fail(rc, "graph_generate: %s", buf);
+}
+
 int albba_graph_layout(albba_ctx *ctx, const albba_graph *G, albba_sellcs **out)
 {
     if (!ctx || !G || !out || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "graph_layout: bad argument");

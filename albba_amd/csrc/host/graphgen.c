@@ -38,7 +38,7 @@ void albba_csr_free(albba_csr *g)
     g->nnz = 0;
 }
 
-static int32_t *make_perm(int32_t n, uint64_t perm_seed)
+int32_t *albba_make_perm(int32_t n, uint64_t perm_seed)
 {
     int32_t *perm = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     if (!perm) return NULL;
@@ -177,7 +177,7 @@ int albba_gen_rmat(int scale, int edgefactor, double a, double b, double c, uint
         src[e] = (int32_t)u;
         dst[e] = (int32_t)v;
     }
-    int32_t *perm = make_perm(n, perm_seed);
+    int32_t *perm = albba_make_perm(n, perm_seed);
     if (!perm) { free(src); free(dst); return albba_fail(ALBBA_ENOMEM, "permutation"); }
     int rc = build_sym_csr(n, ne, src, dst, perm, out);
     free(src); free(dst); free(perm);
@@ -197,7 +197,7 @@ int albba_gen_er(int32_t n, double mean_degree, uint64_t seed, uint64_t perm_see
         src[e] = (int32_t)(albba_rng_at(seed, 2 * (uint64_t)e) % (uint64_t)n);
         dst[e] = (int32_t)(albba_rng_at(seed, 2 * (uint64_t)e + 1) % (uint64_t)n);
     }
-    int32_t *perm = make_perm(n, perm_seed);
+    int32_t *perm = albba_make_perm(n, perm_seed);
     if (!perm) { free(src); free(dst); return albba_fail(ALBBA_ENOMEM, "permutation"); }
     int rc = build_sym_csr(n, ne, src, dst, perm, out);
     free(src); free(dst); free(perm);
@@ -233,7 +233,7 @@ int albba_gen_grid(int32_t rows, int32_t cols, double p_remove, uint64_t seed,
         src[e] = u;
         dst[e] = v;
     }
-    int32_t *perm = make_perm(n, perm_seed);
+    int32_t *perm = albba_make_perm(n, perm_seed);
     if (!perm) { free(src); free(dst); return albba_fail(ALBBA_ENOMEM, "permutation"); }
     int rc = build_sym_csr(n, ne, src, dst, perm, out);
     free(src); free(dst); free(perm);

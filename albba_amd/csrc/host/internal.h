@@ -36,6 +36,10 @@ static inline uint64_t albba_rng_at(uint64_t seed, uint64_t idx)
     return albba_mix64(seed + (idx + 1) * 0x9E3779B97F4A7C15ull);
 }
 
+/* The id permutation of every generator (Fisher-Yates with perm_seed);
+ * malloc'ed, NULL when out of memory. */
+ALBBA_HIDDEN int32_t *albba_make_perm(int32_t n, uint64_t perm_seed);
+
 /* Uniform double in [0,1) from the top 53 bits. */
 static inline double albba_u01(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
 

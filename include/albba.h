@@ -270,6 +270,32 @@ ALBBA_API int albba_gen_grid(int32_t rows, int32_t cols, double p_remove,
  * component (of the symmetric closure).  Returns the number written. */
 ALBBA_API int albba_pick_roots(const albba_csr *g, int count, uint64_t root_seed,
                                int largest_cc, int32_t *roots);
+/* The same graphs generated on the device (bit for bit the CSR of the
+ * albba_gen_* function of that kind), the layout built there as by
+ * albba_graph_build.  host_copy, if not NULL, receives the CSR (free with
+ * albba_csr_free).  The id permutation is drawn on the host (Fisher-Yates is
+ * sequential); edges, permutation, symmetrization, self-loop removal, sort
+ * and dedup run on the device.  No reference counterpart: the reference only
+ * loads files (bfs/src/main.c:88-99). */
+enum { ALBBA_GEN_RMAT = 0, ALBBA_GEN_ER = 1, ALBBA_GEN_GRID = 2 };
+typedef struct albba_gen_params {
+    int32_t kind;                 /* ALBBA_GEN_*                                */
+    int32_t scale, edgefactor;    /* RMAT                                       */
+    double a, b, c;               /* RMAT quadrant probabilities                */
+    int32_t n;                    /* ER vertices                                */
+    int32_t rows, cols;           /* grid                                       */
+    double mean_degree;           /* ER                                         */
+    double p_remove;              /* grid                                       */
+    uint64_t seed, perm_seed;
+} albba_gen_params;
+typedef struct albba_gen_report {
+    float perm_ms;   /* the permutation on the host                         */
+    float gen_ms;    /* edges to CSR on the device                          */
+    float build_ms;  /* the layout on the device                            */
+    float total_ms;
+} albba_gen_report;
+ALBBA_API int albba_graph_generate(albba_ctx *ctx, const albba_gen_params *p, int C, int sigma, int pad,
+                                   albba_graph **out, albba_csr *host_copy, albba_gen_report *report);
 /* Counter-based RNG used by every generator: splitmix64 finaliser of
  * seed + (idx+1) * 0x9E3779B97F4A7C15. */
 ALBBA_API uint64_t albba_rng_u64(uint64_t seed, uint64_t idx);

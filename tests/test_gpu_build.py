@@ -210,3 +210,54 @@ def test_cbd_load_errors(ctx, tmp_path):
         A.DeviceGraph.load(ctx, str(tmp_path / "g.mtx"))
     # the good file still loads on the same context afterwards
     A.DeviceGraph.load(ctx, str(good)).close()
+
+
+GEN_CASES = [
+    ("rmat", dict(scale=10, edgefactor=16, seed=121, perm_seed=122), lambda: A.Graph.rmat(10, 16, seed=121, perm_seed=122)),
+    ("rmat", dict(scale=12, edgefactor=8, a=0.45, b=0.25, c=0.15, seed=123, perm_seed=124),
+     lambda: A.Graph.rmat(12, 8, 0.45, 0.25, 0.15, seed=123, perm_seed=124)),
+    ("rmat", dict(scale=9, edgefactor=32, a=0.65, b=0.15, c=0.15, seed=125, perm_seed=126),
+     lambda: A.Graph.rmat(9, 32, 0.65, 0.15, 0.15, seed=125, perm_seed=126)),
+    ("er", dict(n=5000, mean_degree=6.0, seed=127, perm_seed=128), lambda: A.Graph.er(5000, 6.0, seed=127, perm_seed=128)),
+    ("er", dict(n=3, mean_degree=40.0, seed=129, perm_seed=130), lambda: A.Graph.er(3, 40.0, seed=129, perm_seed=130)),
+    ("grid", dict(rows=30, cols=41, p_remove=0.1, seed=131, perm_seed=132),
+     lambda: A.Graph.grid(30, 41, 0.1, seed=131, perm_seed=132)),
+    ("grid", dict(rows=7, cols=1, p_remove=0.5, seed=133, perm_seed=134),
+     lambda: A.Graph.grid(7, 1, 0.5, seed=133, perm_seed=134)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(GEN_CASES)))
+def test_device_generators_equal_host_generators(ctx, case):
+    """Edges, permutation, symmetrization, self-loop removal, sort and dedup
+    on the device: the host generator's CSR bit for bit, then the layout."""
+    kind, kw, host_fn = GEN_CASES[case]
+    host = host_fn()
+    dg, got = A.DeviceGraph.generate(ctx, kind, 64, 4096, 1, **kw)
+    np.testing.assert_array_equal(got.row_ptr, host.row_ptr)
+    np.testing.assert_array_equal(got.col, host.col)
+    assert (dg.n, dg.nnz) == (host.n, host.nnz)
+    ref = host.sellcs(64, 4096, 1).arrays()
+    lay = dg.layout().arrays()
+    for k, v in ref.items():
+        if v is None:
+            assert lay[k] is None
+        else:
+            np.testing.assert_array_equal(lay[k], v, err_msg=k)
+    if host.nnz:
+        root = int(host.pick_roots(1, 135)[0])
+        o = dg.bfs(root)
+        lv, pa = O.bfs_queue(host.row_ptr, host.col, root)
+        np.testing.assert_array_equal(o["level"], lv)
+        np.testing.assert_array_equal(o["parent"], pa)
+    dg.close()
+
+
+def test_device_generator_errors(ctx):
+    for kind, kw in [("rmat", dict(scale=0)), ("rmat", dict(scale=8, a=0.6, b=0.3, c=0.3)),
+                     ("er", dict(n=0, mean_degree=2.0)), ("grid", dict(rows=0, cols=4)),
+                     ("grid", dict(rows=4, cols=4, p_remove=1.5))]:
+        with pytest.raises(A.AlbbaError):
+            A.DeviceGraph.generate(ctx, kind, **kw)
+    with pytest.raises(A.AlbbaError):
+        A.DeviceGraph.generate(ctx, "rmat", 48, 4096, 1, scale=8)
