@@ -989,9 +989,15 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                     /* a choice 4x slower than the best so far cannot win:
                      * one trial is enough to rank it */
                     if (r == 1 && t[0] > 4.0f * best) { nr = 1; break; }
+                    /* the state after the select kernel, the preparation the
+                     * deployed loop queues behind it for every choice (zero
+                     * next, unvisited-word list, frontier summary), then the
+                     * choice's own kernels -- the only part that is timed, as
+                     * only that part differs between choices when deployed */
                     if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
                         (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))) ||
-                        (rc = albd_event_record(d, 4)) || (rc = run_level(G, &ls, c, level, frontier, 0)) ||
+                        (rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SLOTS)) || (rc = prep_level(G, &ls)) ||
+                        (rc = albd_event_record(d, 4)) || (rc = run_main(G, &ls, c, level, frontier, 0)) ||
                         (rc = albd_event_record(d, 5)) || (rc = albd_event_elapsed(d, 4, 5, &t[r])))
                         return albba_fail(rc, "bfs: %s", albd_error());
                 }
