@@ -228,6 +228,12 @@ enum { ALBD_PULL_ALL = 0, ALBD_PULL_LIST = 1, ALBD_PULL_LIST_CB = 2 };
 ALBD_HIDDEN int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st,
                                   int from_list, int64_t prev_len);
 ALBD_HIDDEN int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* As albd_pull, queued before the level's direction is known: the kernel
+ * returns at once unless the select kernel chose this variant (spec = 1: its
+ * learned choice, 0: the heuristic's) and the frontier is not empty; spec =
+ * -1 is albd_pull. */
+ALBD_HIDDEN int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant,
+                               int32_t level, int64_t items_bound, int spec);
 /* one launch: zero st->next, compact the unvisited-word list (as
  * albd_build_active), and with `summary` build the frontier summary */
 ALBD_HIDDEN int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list,

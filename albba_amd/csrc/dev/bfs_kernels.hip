@@ -38,6 +38,11 @@ struct PullArgs {
     const unsigned long long *nitems;   // LIST variants: its length (device)
     int32_t sum_words32;                // CB: LDS summary size in 32-bit words
     int32_t per_block;                  // consecutive items per block
+    // queued before the level's direction is known: -1 no; else run only if
+    // the select kernel's choice (1: the learned one, 0: the heuristic's) is
+    // spec_choice and the frontier is not empty
+    int32_t spec;
+    int32_t spec_choice;
 };
 
 __device__ __forceinline__ void slot_add(albd_slot *slots, unsigned long long cnt,
@@ -65,16 +70,16 @@ constexpr int kCBBlock = 1024;
 
 template <int VAR> constexpr int pull_block() { return VAR == ALBD_PULL_LIST_CB ? kCBBlock : kBlock; }
 
-template <int C, int VAR>
-__global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
+// The pull of one level (NT threads a block).
+template <int C, int VAR, int NT>
+__device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
 {
-    extern __shared__ uint32_t lds_sum[];
     const albd_graph &g = a.g;
     const int lane = lane_id();
 
     if constexpr (VAR == ALBD_PULL_LIST_CB) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(a.st.summary);
-        for (int i = threadIdx.x; i < a.sum_words32; i += pull_block<VAR>()) lds_sum[i] = src[i];
+        for (int i = threadIdx.x; i < a.sum_words32; i += NT) lds_sum[i] = src[i];
         __syncthreads();
     }
     const int64_t nitems = (VAR == ALBD_PULL_ALL) ? (int64_t)g.nwords : (int64_t)(*a.nitems);
@@ -257,6 +262,21 @@ __global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
     edges = wave_sum(edges);
     if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp);
 }
+
+template <int C, int VAR>
+__global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
+{
+    extern __shared__ uint32_t lds_sum[];
+    if (a.spec >= 0) {
+        // the same counters for every thread: a uniform early return
+        const albd_counters *ctr = a.st.ctr;
+        const int go = __builtin_amdgcn_readfirstlane(
+            ctr->frontier != 0 && (a.spec ? ctr->sel_choice : ctr->heur_choice) == a.spec_choice ? 1 : 0);
+        if (!go) return;
+    }
+    pull_body<C, VAR, pull_block<VAR>()>(a, lds_sum);
+}
+
 
 // ---------------------------------------------------------------------------
 // word passes
@@ -1218,10 +1238,19 @@ int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st)
 int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant, int32_t level,
               int64_t items_bound)
 {
+    return albd_pull_spec(c, g, st, variant, level, items_bound, -1);
+}
+
+int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant, int32_t level,
+                   int64_t items_bound, int spec)
+{
     PullArgs a;
     a.g = *g;
     a.st = *st;
     a.level = level;
+    a.spec = spec;
+    a.spec_choice = variant == ALBD_PULL_ALL ? ALBBA_CHOICE_PULL
+                                             : (variant == ALBD_PULL_LIST ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PULL_CB);
     a.items = st->active_next;
     a.nitems = &st->ctr->active_next;
     a.per_block = 16;

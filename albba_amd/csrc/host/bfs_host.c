@@ -720,7 +720,7 @@ typedef struct loop_state {
 
 typedef struct level_sums {
     int64_t next_count, next_edges, next_words, scanned, words, bypassed;
-    int64_t qnext_len, active_next, queue_len, push_edges;
+    int64_t qnext_len, active_next, active_len, queue_len, push_edges;
 } level_sums;
 
 static void sum_counters(const albd_counters *h, level_sums *o)
@@ -734,6 +734,7 @@ static void sum_counters(const albd_counters *h, level_sums *o)
     o->bypassed = (int64_t)h->bypassed;
     o->qnext_len = (int64_t)h->qnext_len;
     o->active_next = (int64_t)h->active_next;
+    o->active_len = (int64_t)h->active_len;
     o->queue_len = (int64_t)h->queue_len;
     o->push_edges = (int64_t)h->push_edges;
 }
@@ -879,9 +880,11 @@ static void commit_pointers(albba_graph *G, loop_state *ls, int choice)
     }
 }
 
+/* After the select kernel (which hands a used unvisited-word list over as
+ * the current one): the current list's length bounds the next lists. */
 static void commit_counts(loop_state *ls, int choice, const level_sums *sums)
 {
-    if (choice == ALBBA_CHOICE_PULL_SB || choice == ALBBA_CHOICE_PULL_CB) ls->list_len = sums->active_next;
+    if (choice == ALBBA_CHOICE_PULL_SB || choice == ALBBA_CHOICE_PULL_CB) ls->list_len = sums->active_len;
 }
 
 static int choice_ok(const albba_graph *G, int c)
@@ -951,6 +954,8 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * after their choice (run_level). */
     const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED;
     unsigned int seq = 0;
+    /* the choice whose pull is already queued for the coming level, or -1 */
+    int queued_pull = -1;
     /* init, then the select kernel turns the root into level 0's frontier */
     if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
@@ -1028,8 +1033,9 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         if (small)
             rc = run_small(G, &ls, level, frontier, opts.small_frontier_edges, push_vertex_cap,
                            opts.small_frontier_blocks, opts.time_levels, spec, &nlev);
-        else if (spec)
-            rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
+        else if (spec) {
+            if (choice != queued_pull) rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
+        }
         else
             rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
         if (rc) return albba_fail(rc, "bfs: %s", albd_error());
@@ -1045,9 +1051,28 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
              * the next level's preparation behind it, then wait for the
              * counters */
             commit_pointers(G, &ls, choice);
-            if ((rc = albd_select_post(d, &G->g, &G->st, &sel, &seq)) || (rc = prep_level(G, &ls)) ||
-                (rc = albd_mail_wait(d, &G->st, seq)))
+            if ((rc = albd_select_post(d, &G->g, &G->st, &sel, &seq)) || (rc = prep_level(G, &ls)))
                 return albba_fail(rc, "bfs: %s", albd_error());
+            /* After a pull the next level takes the same pull, as a rule:
+             * queue it now, so that the device does not wait for the host to
+             * read the choice.  The kernel reads the choice itself and returns
+             * at once if it differs; the host then queues the chosen kernels
+             * behind it.  Not with per-level events, which must bracket the
+             * chosen kernels. */
+            queued_pull = -1;
+            if (choice != ALBBA_CHOICE_PUSH && !opts.time_levels) {
+                const int variant = choice == ALBBA_CHOICE_PULL_SB
+                                        ? ALBD_PULL_LIST
+                                        : (choice == ALBBA_CHOICE_PULL_CB ? ALBD_PULL_LIST_CB : ALBD_PULL_ALL);
+                /* lists only shrink: the last length the host knows bounds
+                 * the list prep_level builds */
+                if ((rc = albd_pull_spec(d, &G->g, &G->st, variant, level + nlev,
+                                         ls.list_valid && ls.list_len > 0 ? ls.list_len : G->g.nwords,
+                                         policy == ALBBA_SWITCH_LEARNED)))
+                    return albba_fail(rc, "bfs: %s", albd_error());
+                queued_pull = choice;
+            }
+            if ((rc = albd_mail_wait(d, &G->st, seq))) return albba_fail(rc, "bfs: %s", albd_error());
         } else if ((rc = albd_select_read(d, &G->g, &G->st, &sel))) {
             return albba_fail(rc, "bfs: %s", albd_error());
         }
