@@ -37,8 +37,7 @@ typedef struct albd_slot {
 typedef struct albd_counters {
     unsigned long long queue_len;    /* current frontier queue length         */
     unsigned long long qnext_len;    /* queue being built                     */
-    unsigned long long active_len;   /* active (unvisited) word list length   */
-    unsigned long long active_next;  /* list being built                      */
+    unsigned long long active_len;   /* length of the current unvisited-word list */
     unsigned long long push_edges;   /* sum of degrees over the queue         */
     /* written by the select kernel at the end of every level */
     unsigned long long frontier;       /* vertices found by the last level     */
@@ -58,6 +57,13 @@ typedef struct albd_counters {
     int pad_;
     unsigned int seq;                  /* host mailbox: sequence of the copy    */
     unsigned int pad2_;
+    /* Unvisited-word list lengths, slot level % 3: built by that level's
+     * preparation, read by its pull, handed over (to active_len) by the next
+     * select kernel, which also zeroes slot (level + 2) % 3 for the level
+     * after.  Three slots let the select kernel of one level and the
+     * preparation of the next run in one launch.  Outside the block the
+     * select kernel writes back. */
+    unsigned long long list_cnt[3];
 } albd_counters;
 
 /* levels one small-frontier call may run (st.level_log holds 4 per level) */
@@ -223,10 +229,12 @@ ALBD_HIDDEN int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_st
 
 /* pull variants */
 enum { ALBD_PULL_ALL = 0, ALBD_PULL_LIST = 1, ALBD_PULL_LIST_CB = 2 };
-/* compact the unvisited-word list: from all words (from_list = 0) or from the
- * previous list (from_list = 1); result in st->active, length in ctr->active_len */
+/* compact the unvisited-word list of `level`: from all words (from_list = 0)
+ * or from the previous list st->active of length ctr->active_len (from_list =
+ * 1; prev_len bounds it on the host); result in st->active_next, length in
+ * ctr->list_cnt[level % 3] */
 ALBD_HIDDEN int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st,
-                                  int from_list, int64_t prev_len);
+                                  int from_list, int64_t prev_len, int32_t level);
 ALBD_HIDDEN int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st);
 /* As albd_pull, queued before the level's direction is known: the kernel
  * returns at once unless the select kernel chose this variant (spec = 1: its
@@ -237,7 +245,12 @@ ALBD_HIDDEN int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_stat
 /* one launch: zero st->next, compact the unvisited-word list (as
  * albd_build_active), and with `summary` build the frontier summary */
 ALBD_HIDDEN int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list,
-                          int summary);
+                          int summary, int32_t level);
+/* albd_select_post and albd_prep of level sel->level in one launch (the
+ * select kernel in one block, the preparation in the others) */
+ALBD_HIDDEN int albd_select_prep_post(albd_ctx *c, const albd_graph *g, const albd_state *st,
+                                      const albd_select_args *sel, unsigned int *seq, int from_list,
+                                      int summary);
 ALBD_HIDDEN int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st, int variant,
                           int32_t level, int64_t items_bound);
 ALBD_HIDDEN int albd_cb_supported(const albd_graph *g);

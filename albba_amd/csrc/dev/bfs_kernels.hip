@@ -364,8 +364,8 @@ __global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 // level.  One block.
 constexpr int kSelectBlock = 256;
 
-__global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel,
-                                                         albd_counters *mail, unsigned int seq)
+__device__ __forceinline__ void select_body(const albd_graph &g, const albd_state &st, const albd_select_args &sel,
+                                            albd_counters *mail, unsigned int seq)
 {
     // the counter block up to the mailbox sequence word, as 64-bit words
     constexpr int kCtrWords = (int)(offsetof(albd_counters, seq) / 8);
@@ -386,8 +386,15 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
     constexpr int kPer = ALBD_NSLOT / kSelectBlock;
     static_assert(ALBD_NSLOT % kSelectBlock == 0, "slots split evenly over the block");
     static_assert(sizeof(albd_slot) == 64, "slot = 3 x 16 counter bytes + padding");
-    unsigned long long cw = 0;
+    // a used unvisited-word list (slot (level - 1) % 3, built by the
+    // previous level's preparation) becomes the current list
+    const bool list_used = !sel.first && (sel.prev_choice == ALBBA_CHOICE_PULL_SB ||
+                                          sel.prev_choice == ALBBA_CHOICE_PULL_CB);
+    unsigned long long cw = 0, used_len = 0;
     if (tid < kCtrWords) cw = ctr_words[tid];
+    if (tid == 0 && list_used) used_len = st.ctr->list_cnt[(sel.level + 2) % 3];
+    // the slot that the level after this one fills starts from zero
+    if (tid == 0) st.ctr->list_cnt[(sel.level + 1) % 3] = 0;
     float pr[mlp_per_thread<kSelectBlock>()];
     if (sel.params) mlp_load<kSelectBlock>(pr, sel.params);
     ulonglong2 v[kPer][3];
@@ -469,9 +476,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         // the bitmap, from zero); a used unvisited-word list becomes the list
         if (!sel.first) c.queue_len = sel.prev_choice == ALBBA_CHOICE_PUSH ? c.qnext_len : 0ull;
         c.qnext_len = 0;
-        if (sel.prev_choice == ALBBA_CHOICE_PULL_SB || sel.prev_choice == ALBBA_CHOICE_PULL_CB)
-            c.active_len = c.active_next;
-        c.active_next = 0;
+        if (list_used) c.active_len = used_len;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -487,6 +492,12 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_stat
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+}
+
+__global__ __launch_bounds__(kSelectBlock) void k_select(albd_graph g, albd_state st, albd_select_args sel,
+                                                         albd_counters *mail, unsigned int seq)
+{
+    select_body(g, st, sel, mail, seq);
 }
 
 // Zero st.next (the select kernel has moved the counters and zeroed the
@@ -514,22 +525,36 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
                 st.persist->error = 0;
                 for (int f = 0; f < 4; ++f) st.level_log[f] = 0;
                 st.ctr->persist_error = 0;
+                // the levels run inside that kernel skip the select kernels
+                // that clear the list-count slots
+                for (int k = 0; k < 3; ++k) st.ctr->list_cnt[k] = 0;
             }
         }
     }
 }
 
+// Where the list preparation of level L finds its counts: the input list's
+// length in active_len (in_slot < 0: the select kernel of level L has
+// already handed it over) or in list_cnt[in_slot] (the select kernel runs in
+// the same launch); the output count in list_cnt[out_slot = L % 3].
+struct ListSlots {
+    int32_t in_slot, out_slot;
+};
+
 // unvisited-word list: from every word (FROM_LIST = false) or from the
-// previous list (visited only grows, so an old list is a superset)
+// previous list (visited only grows, so an old list is a superset).  Blocks
+// bid of nb.
 template <bool FROM_LIST>
-__device__ __forceinline__ void active_body(const albd_graph &g, const albd_state &st,
-                                            unsigned int *s_cnt, unsigned long long *s_base)
+__device__ __forceinline__ void active_body(const albd_graph &g, const albd_state &st, ListSlots ls, int bid,
+                                            int nb, unsigned int *s_cnt, unsigned long long *s_base)
 {
     constexpr int kGroups = 4; // 64-item groups per wave per step
     const int lane = lane_id(), wid = threadIdx.x >> 6;
-    const int64_t n_in = FROM_LIST ? (int64_t)st.ctr->active_len : (int64_t)g.nwords;
+    const int64_t n_in =
+        FROM_LIST ? (int64_t)(ls.in_slot < 0 ? st.ctr->active_len : st.ctr->list_cnt[ls.in_slot]) : (int64_t)g.nwords;
+    unsigned long long *out_cnt = &st.ctr->list_cnt[ls.out_slot];
     const int64_t step = (int64_t)kBlock * kGroups;
-    for (int64_t base = (int64_t)blockIdx.x * step; base < n_in; base += (int64_t)gridDim.x * step) {
+    for (int64_t base = (int64_t)bid * step; base < n_in; base += (int64_t)nb * step) {
         int32_t w[kGroups];
         uint64_t m[kGroups];
         unsigned int cnt = 0;
@@ -545,7 +570,7 @@ __device__ __forceinline__ void active_body(const albd_graph &g, const albd_stat
             m[k] = __ballot(keep);
             cnt += (unsigned int)__popcll(m[k]);
         }
-        unsigned long long off = block_reserve(&st.ctr->active_next, cnt, s_cnt, s_base);
+        unsigned long long off = block_reserve(out_cnt, cnt, s_cnt, s_base);
 #pragma unroll
         for (int k = 0; k < kGroups; ++k) {
             if ((m[k] >> lane) & 1ull) st.active_next[off + __popcll(m[k] & lanemask_lt(lane))] = w[k];
@@ -554,12 +579,12 @@ __device__ __forceinline__ void active_body(const albd_graph &g, const albd_stat
     }
 }
 
-__device__ __forceinline__ void summary_body(const albd_graph &g, const albd_state &st)
+__device__ __forceinline__ void summary_body(const albd_graph &g, const albd_state &st, int bid, int nb)
 {
     const int lane = lane_id();
     const int64_t ngroups = ((int64_t)g.nwords + 63) >> 6;
-    const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+    const int64_t gw = ((int64_t)bid * kBlock + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)nb * kBlock) >> 6;
     for (int64_t grp = gw; grp < ngroups; grp += nw) {
         const int64_t w = grp * 64 + lane;
         const bool nz = (w < g.nwords) && st.front[w] != 0;
@@ -569,31 +594,51 @@ __device__ __forceinline__ void summary_body(const albd_graph &g, const albd_sta
 }
 
 template <bool FROM_LIST>
-__global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st)
+__global__ __launch_bounds__(kBlock) void k_active(albd_graph g, albd_state st, ListSlots ls)
 {
     __shared__ unsigned int s_cnt[kWavesPerBlock];
     __shared__ unsigned long long s_base;
-    active_body<FROM_LIST>(g, st, s_cnt, &s_base);
+    active_body<FROM_LIST>(g, st, ls, blockIdx.x, gridDim.x, s_cnt, &s_base);
 }
 
 __global__ __launch_bounds__(kBlock) void k_summary(albd_graph g, albd_state st)
 {
-    summary_body(g, st);
+    summary_body(g, st, blockIdx.x, gridDim.x);
 }
 
-// The next level's preparation in one launch: zero st.next, compact the
-// unvisited-word list, build the frontier summary.  The three parts touch
-// disjoint data; the counters they use were set by the select kernel.
+// The next level's preparation: zero st.next, compact the unvisited-word
+// list, build the frontier summary.  The three parts touch disjoint data.
+// Blocks bid of nb.
 template <bool FROM_LIST, bool SUMMARY>
-__global__ __launch_bounds__(kBlock) void k_prep(albd_graph g, albd_state st)
+__device__ __forceinline__ void prep_body(const albd_graph &g, const albd_state &st, ListSlots ls, int bid, int nb)
 {
     __shared__ unsigned int s_cnt[kWavesPerBlock];
     __shared__ unsigned long long s_base;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < g.nwords;
-         i += (int64_t)gridDim.x * kBlock)
+    for (int64_t i = (int64_t)bid * kBlock + threadIdx.x; i < g.nwords; i += (int64_t)nb * kBlock)
         st.next[i] = 0;
-    active_body<FROM_LIST>(g, st, s_cnt, &s_base);
-    if constexpr (SUMMARY) summary_body(g, st);
+    active_body<FROM_LIST>(g, st, ls, bid, nb, s_cnt, &s_base);
+    if constexpr (SUMMARY) summary_body(g, st, bid, nb);
+}
+
+template <bool FROM_LIST, bool SUMMARY>
+__global__ __launch_bounds__(kBlock) void k_prep(albd_graph g, albd_state st, ListSlots ls)
+{
+    prep_body<FROM_LIST, SUMMARY>(g, st, ls, blockIdx.x, gridDim.x);
+}
+
+// The select kernel of level L (block 0) and level L's preparation (the
+// other blocks) in one launch, so that neither waits for the other.  They
+// share no data: the preparation reads the input list's length from its
+// slot rather than from active_len, writes its count into a slot the select
+// kernel does not touch, and zeroes st.next, which the select kernel does
+// not read; the counters the select kernel sums are in st.slots.
+template <bool FROM_LIST, bool SUMMARY>
+__global__ __launch_bounds__(kBlock) void k_select_prep(albd_graph g, albd_state st, albd_select_args sel,
+                                                        albd_counters *mail, unsigned int seq, ListSlots ls)
+{
+    static_assert(kSelectBlock == kBlock, "one block shape for both parts");
+    if (blockIdx.x == 0) select_body(g, st, sel, mail, seq);
+    else prep_body<FROM_LIST, SUMMARY>(g, st, ls, blockIdx.x - 1, gridDim.x - 1);
 }
 
 // Words -> queue of positions with out-degree > 0.  FINALIZE (after a push):
@@ -1192,15 +1237,16 @@ int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_state *st)
 }
 
 int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st, int from_list,
-                      int64_t prev_len)
+                      int64_t prev_len, int32_t level)
 {
     hipStream_t s = albd_stream(c);
     const int64_t n_in = from_list ? prev_len : g->nwords;
     const int grid = grid_for_groups(c, (n_in + 63) / 64);
+    const ListSlots ls{-1, level % 3};
     if (from_list)
-        hipLaunchKernelGGL(k_active<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        hipLaunchKernelGGL(k_active<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
     else
-        hipLaunchKernelGGL(k_active<false>, dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        hipLaunchKernelGGL(k_active<false>, dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1212,16 +1258,43 @@ int albd_cb_supported(const albd_graph *g)
     return bytes <= 64 * 1024;
 }
 
-int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list, int summary)
+int albd_prep(albd_ctx *c, const albd_graph *g, const albd_state *st, int from_list, int summary, int32_t level)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
     hipStream_t s = albd_stream(c);
+    const ListSlots ls{-1, level % 3};
     if (from_list) {
-        if (summary) hipLaunchKernelGGL((k_prep<true, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
-        else hipLaunchKernelGGL((k_prep<true, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        if (summary) hipLaunchKernelGGL((k_prep<true, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
+        else hipLaunchKernelGGL((k_prep<true, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
     } else {
-        if (summary) hipLaunchKernelGGL((k_prep<false, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
-        else hipLaunchKernelGGL((k_prep<false, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st);
+        if (summary) hipLaunchKernelGGL((k_prep<false, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
+        else hipLaunchKernelGGL((k_prep<false, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st, ls);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_select_prep_post(albd_ctx *c, const albd_graph *g, const albd_state *st, const albd_select_args *sel,
+                          unsigned int *seq, int from_list, int summary)
+{
+    albd_counters *mail = albd_mail_next(c, seq);
+    // one select block in front of the preparation's grid
+    const int grid = 1 + grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
+    hipStream_t s = albd_stream(c);
+    const bool list_used = !sel->first && (sel->prev_choice == ALBBA_CHOICE_PULL_SB ||
+                                           sel->prev_choice == ALBBA_CHOICE_PULL_CB);
+    const ListSlots ls{list_used ? (sel->level + 2) % 3 : -1, sel->level % 3};
+    const unsigned int q = *seq;
+    if (from_list) {
+        if (summary)
+            hipLaunchKernelGGL((k_select_prep<true, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st, *sel, mail, q, ls);
+        else
+            hipLaunchKernelGGL((k_select_prep<true, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st, *sel, mail, q, ls);
+    } else {
+        if (summary)
+            hipLaunchKernelGGL((k_select_prep<false, true>), dim3(grid), dim3(kBlock), 0, s, *g, *st, *sel, mail, q, ls);
+        else
+            hipLaunchKernelGGL((k_select_prep<false, false>), dim3(grid), dim3(kBlock), 0, s, *g, *st, *sel, mail, q, ls);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -1251,8 +1324,9 @@ int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int v
     a.spec = spec;
     a.spec_choice = variant == ALBD_PULL_ALL ? ALBBA_CHOICE_PULL
                                              : (variant == ALBD_PULL_LIST ? ALBBA_CHOICE_PULL_SB : ALBBA_CHOICE_PULL_CB);
+    // the list level `level`'s preparation built, and its count
     a.items = st->active_next;
-    a.nitems = &st->ctr->active_next;
+    a.nitems = &st->ctr->list_cnt[level % 3];
     a.per_block = 16;
     const int64_t sum_words64 = ((int64_t)g->nwords + 63) / 64;
     a.sum_words32 = (int32_t)(sum_words64 * 2);

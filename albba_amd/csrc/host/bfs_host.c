@@ -720,7 +720,7 @@ typedef struct loop_state {
 
 typedef struct level_sums {
     int64_t next_count, next_edges, next_words, scanned, words, bypassed;
-    int64_t qnext_len, active_next, active_len, queue_len, push_edges;
+    int64_t qnext_len, active_len, queue_len, push_edges;
 } level_sums;
 
 static void sum_counters(const albd_counters *h, level_sums *o)
@@ -733,7 +733,6 @@ static void sum_counters(const albd_counters *h, level_sums *o)
     o->words = (int64_t)h->words;
     o->bypassed = (int64_t)h->bypassed;
     o->qnext_len = (int64_t)h->qnext_len;
-    o->active_next = (int64_t)h->active_next;
     o->active_len = (int64_t)h->active_len;
     o->queue_len = (int64_t)h->queue_len;
     o->push_edges = (int64_t)h->push_edges;
@@ -760,7 +759,7 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
     if (choice == ALBBA_CHOICE_PULL_SB) variant = ALBD_PULL_LIST;
     if (choice == ALBBA_CHOICE_PULL_CB) variant = ALBD_PULL_LIST_CB;
     if (variant != ALBD_PULL_ALL &&
-        (rc = albd_build_active(d, &G->g, &G->st, ls->list_valid, ls->list_len)))
+        (rc = albd_build_active(d, &G->g, &G->st, ls->list_valid, ls->list_len, level)))
         return rc;
     if (variant == ALBD_PULL_LIST_CB && (rc = albd_build_summary(d, &G->g, &G->st))) return rc;
     if (mark && (rc = albd_event_record(d, 6))) return rc;
@@ -775,10 +774,17 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
  * `next` and the slots, move the queue/list counters, compact the
  * unvisited-word list (slice bypass) and build the frontier summary (column
  * bypass).  A list that the chosen kernel does not use is dropped. */
-static int prep_level(albba_graph *G, const loop_state *ls)
+static int prep_level(albba_graph *G, const loop_state *ls, int32_t level)
 {
     /* one launch sized for every word (the list length is on the device) */
-    return albd_prep(G->ctx->d, &G->g, &G->st, ls->list_valid, albd_cb_supported(&G->g));
+    return albd_prep(G->ctx->d, &G->g, &G->st, ls->list_valid, albd_cb_supported(&G->g), level);
+}
+
+/* The select kernel of level sel->level and that level's preparation
+ * (prep_level) in one launch; the counters come to the mailbox (seq). */
+static int select_prep(albba_graph *G, const loop_state *ls, const albd_select_args *sel, unsigned int *seq)
+{
+    return albd_select_prep_post(G->ctx->d, &G->g, &G->st, sel, seq, ls->list_valid, albd_cb_supported(&G->g));
 }
 
 /* The kernels of one level after prep_level (no pointer swaps). */
@@ -950,7 +956,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     int32_t level = 0;
     int rc;
     /* The heuristic and learned policies prepare every level speculatively
-     * behind the select kernel (prep_level); the others prepare each level
+     * beside the select kernel (select_prep); the others prepare each level
      * after their choice (run_level). */
     const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED;
     unsigned int seq = 0;
@@ -960,8 +966,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (spec)
-        rc = albd_select_post(d, &G->g, &G->st, &sel, &seq) || prep_level(G, &ls) ||
-             albd_mail_wait(d, &G->st, seq);
+        rc = select_prep(G, &ls, &sel, &seq) || albd_mail_wait(d, &G->st, seq);
     else
         rc = albd_select_read(d, &G->g, &G->st, &sel);
     if (rc) return albba_fail(ALBBA_EHIP, "bfs: %s", albd_error());
@@ -1001,7 +1006,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                      * only that part differs between choices when deployed */
                     if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
                         (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))) ||
-                        (rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SLOTS)) || (rc = prep_level(G, &ls)) ||
+                        (rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SLOTS)) || (rc = prep_level(G, &ls, level)) ||
                         (rc = albd_event_record(d, 4)) || (rc = run_main(G, &ls, c, level, frontier, 0)) ||
                         (rc = albd_event_record(d, 5)) || (rc = albd_event_elapsed(d, 4, 5, &t[r])))
                         return albba_fail(rc, "bfs: %s", albd_error());
@@ -1047,12 +1052,11 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         sel.prev_choice = choice;
         sel.first = 0;
         if (spec) {
-            /* swap to the next level's state, post the select kernel, queue
-             * the next level's preparation behind it, then wait for the
+            /* swap to the next level's state, post the select kernel with
+             * the next level's preparation beside it, then wait for the
              * counters */
             commit_pointers(G, &ls, choice);
-            if ((rc = albd_select_post(d, &G->g, &G->st, &sel, &seq)) || (rc = prep_level(G, &ls)))
-                return albba_fail(rc, "bfs: %s", albd_error());
+            if ((rc = select_prep(G, &ls, &sel, &seq))) return albba_fail(rc, "bfs: %s", albd_error());
             /* After a pull the next level takes the same pull, as a rule:
              * queue it now, so that the device does not wait for the host to
              * read the choice.  The kernel reads the choice itself and returns
@@ -1065,7 +1069,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                                         ? ALBD_PULL_LIST
                                         : (choice == ALBBA_CHOICE_PULL_CB ? ALBD_PULL_LIST_CB : ALBD_PULL_ALL);
                 /* lists only shrink: the last length the host knows bounds
-                 * the list prep_level builds */
+                 * the list the preparation builds */
                 if ((rc = albd_pull_spec(d, &G->g, &G->st, variant, level + nlev,
                                          ls.list_valid && ls.list_len > 0 ? ls.list_len : G->g.nwords,
                                          policy == ALBBA_SWITCH_LEARNED)))
