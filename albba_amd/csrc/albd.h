@@ -90,6 +90,8 @@ typedef struct albd_select {
 typedef struct albd_graph {
     int32_t n, C, pad, nslices, nwords;
     int32_t sparse_lanes;       /* pull: at most this many live lanes -> row phase */
+    int32_t win;                /* positions [k*win, (k+1)*win) hold the vertices of
+                                 * the same id range (the sigma windows) */
     int64_t npos, nnz, padded_nnz, nvops;
     const int64_t *slice_ptr;   /* [nslices+1] */
     const int32_t *slice_width; /* [nslices]   */

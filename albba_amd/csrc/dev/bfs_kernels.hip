@@ -315,48 +315,42 @@ __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint6
     if (lane_id() == 0 && w < nwords) haslen[w] = m;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_state st, int32_t n, int64_t npos,
-                                                      int32_t nwords)
+// The state of a BFS from position rp (vertex ro) in one launch: the bitmaps
+// and the slot and counter blocks zeroed, except what the root sets -- its
+// visited and frontier bits, its level/parent entry, the queue holding it
+// (when it has out-edges), and slot 0 seeded as the frontier of "level -1",
+// which the first select kernel sums.  lp holds stale values outside the
+// visited set (the vertex-order pass masks them), and pmin is all ones at
+// rest (the push finalize restores every entry it consumed).
+__global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 {
-    // lp holds stale values outside the visited set (the
-    // vertex-order pass masks them), and pmin is all ones at rest (the push
-    // finalize restores every entry it consumed), so only the bitmaps reset
-    (void)n;
-    (void)npos;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords;
+    const int64_t deg = g.push_ptr[rp + 1] - g.push_ptr[rp];
+    const int32_t rw = rp >> 6;
+    const uint64_t bit = 1ull << (rp & 63);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < g.nwords;
          i += (int64_t)gridDim.x * kBlock) {
-        st.visited[i] = 0;
-        st.front[i] = 0;
+        const uint64_t v = i == rw ? bit : 0ull;
+        st.visited[i] = v;
+        st.front[i] = v;
         st.next[i] = 0;
     }
+    static_assert(offsetof(albd_slot, next_count) == 0 && offsetof(albd_slot, next_edges) == 8 &&
+                      offsetof(albd_slot, next_words) == 16,
+                  "slot 0 seeded by word");
     unsigned long long *slots = reinterpret_cast<unsigned long long *>(st.slots);
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
          i < (int64_t)(ALBD_NSLOT * sizeof(albd_slot) / 8); i += (int64_t)gridDim.x * kBlock)
-        slots[i] = 0;
+        slots[i] = i == 0 || i == 2 ? 1ull : (i == 1 ? (unsigned long long)deg : 0ull);
     if (blockIdx.x == 0) {
+        constexpr int kQ = (int)(offsetof(albd_counters, queue_len) / 8);
         unsigned long long *c = reinterpret_cast<unsigned long long *>(st.ctr);
-        for (int i = threadIdx.x; i < (int)(sizeof(albd_counters) / 8); i += kBlock) c[i] = 0;
+        for (int i = threadIdx.x; i < (int)(sizeof(albd_counters) / 8); i += kBlock)
+            c[i] = i == kQ && deg > 0 ? 1ull : 0ull;
+        if (threadIdx.x == 0) {
+            st.lp[rp] = pack_lp(0, ro);
+            if (deg > 0) st.queue[0] = rp;
+        }
     }
-}
-
-__global__ void k_bfs_root(albd_graph g, albd_state st, int32_t rp, int32_t ro)
-{
-    st.lp[rp] = pack_lp(0, ro);
-    const uint64_t bit = 1ull << (rp & 63);
-    st.visited[rp >> 6] = bit;
-    st.front[rp >> 6] = bit;
-    const int64_t deg = g.push_ptr[rp + 1] - g.push_ptr[rp];
-    if (deg > 0) {
-        st.queue[0] = rp;
-        st.ctr->queue_len = 1;
-    } else {
-        st.ctr->queue_len = 0;
-    }
-    // the root is the frontier of "level -1": seed the slot counters the
-    // select kernel sums
-    st.slots[0].next_count = 1;
-    st.slots[0].next_edges = (unsigned long long)deg;
-    st.slots[0].next_words = 1;
 }
 
 // End of a level: the frontier it produced, the running totals, the
@@ -710,10 +704,60 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
     }
 }
 
-// level and parent from position order to vertex order, once per BFS.  A
-// block takes 4096 consecutive vertex ids: with sigma windows of 4096 rows
-// their positions form one window, so the gathers stay inside one 32 KB
-// stretch of lp read through one XCD's L2, and the writes are coalesced.
+// level and parent from position order to vertex order, once per BFS.
+// Positions [k*win, (k+1)*win) hold the vertices of the same id range (the
+// sigma windows), so a block takes one window: it streams the window's
+// row_order, lp and visited words in position order, scatters level and
+// parent into LDS by vertex, and writes both out in vertex order.  Every
+// global access is a coalesced stream.  LDS: 8 bytes per window row.
+constexpr int kMaxWin = 8192;
+
+__global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, albd_state st)
+{
+    extern __shared__ int32_t s_win[];
+    int32_t *s_level = s_win, *s_parent = s_win + g.win;
+    constexpr int kV = 4; // positions in flight per thread
+    const int64_t nwin = ((int64_t)g.n + g.win - 1) / g.win;
+    for (int64_t w = blockIdx.x; w < nwin; w += gridDim.x) {
+        const int64_t lo = w * g.win;
+        const int32_t len = (int32_t)((int64_t)g.n - lo < g.win ? (int64_t)g.n - lo : g.win);
+        for (int32_t i0 = threadIdx.x; i0 < len; i0 += kBlock * kV) {
+            int32_t v[kV];
+            unsigned long long x[kV];
+            uint64_t vis[kV];
+#pragma unroll
+            for (int k = 0; k < kV; ++k) {
+                const int32_t i = i0 + k * kBlock;
+                const int64_t p = lo + (i < len ? i : 0);
+                v[k] = g.row_order[p];
+                x[k] = st.lp[p];
+                vis[k] = st.visited[p >> 6];
+            }
+#pragma unroll
+            for (int k = 0; k < kV; ++k) {
+                const int32_t i = i0 + k * kBlock;
+                if (i < len) {
+                    const int64_t p = lo + i;
+                    const bool reached = (vis[k] >> (p & 63)) & 1ull;
+                    const int32_t o = (int32_t)(v[k] - lo);
+                    if ((uint32_t)o < (uint32_t)len) {
+                        s_level[o] = reached ? (int32_t)(x[k] >> 32) : -1;
+                        s_parent[o] = reached ? (int32_t)(uint32_t)x[k] : -1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < len; i += kBlock) {
+            st.level[lo + i] = s_level[i];
+            st.parent[lo + i] = s_parent[i];
+        }
+        __syncthreads(); // the LDS arrays are reused by the next window
+    }
+}
+
+// The same for windows longer than the LDS arrays: a block takes 4096
+// consecutive vertex ids and gathers lp through pos_of.
 constexpr int kVertexChunk = 4096;
 
 __global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st)
@@ -1200,19 +1244,25 @@ int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen)
 int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t rp, int32_t ro)
 {
     hipStream_t s = albd_stream(c);
-    int64_t total = g->npos > g->n ? g->npos : g->n;
-    int64_t blocks = (total + kBlock - 1) / kBlock;
+    int64_t blocks = ((int64_t)g->nwords + kBlock - 1) / kBlock;
     int64_t cap = (int64_t)albd_num_cus(c) * 16;
-    hipLaunchKernelGGL(k_bfs_reset, dim3((unsigned)(blocks > cap ? cap : blocks)), dim3(kBlock), 0, s,
-                       *st, g->n, g->npos, g->nwords);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_bfs_root, dim3(1), dim3(1), 0, s, *g, *st, rp, ro);
+    hipLaunchKernelGGL(k_bfs_reset, dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))), dim3(kBlock),
+                       0, s, *g, *st, rp, ro);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
 int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
+    if (g->win >= 1 && g->win <= kMaxWin) {
+        int64_t blocks = ((int64_t)g->n + g->win - 1) / g->win;
+        int64_t cap = (int64_t)albd_num_cus(c) * 16;
+        hipLaunchKernelGGL(k_to_vertex_order_win,
+                           dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))), dim3(kBlock),
+                           sizeof(int32_t) * 2 * (size_t)g->win, albd_stream(c), *g, *st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     int64_t blocks = ((int64_t)g->n + kVertexChunk - 1) / kVertexChunk;
     int64_t cap = (int64_t)albd_num_cus(c) * 32;
     hipLaunchKernelGGL(k_to_vertex_order, dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))),

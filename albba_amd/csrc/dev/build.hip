@@ -554,6 +554,7 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     g->C = C;
     g->pad = pad;
     g->sparse_lanes = 8;
+    g->win = sigma > 0 ? sigma : 1;
     g->nslices = (int32_t)ns;
     g->nwords = (int32_t)nwords;
     g->npos = npos;

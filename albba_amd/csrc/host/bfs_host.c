@@ -239,6 +239,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     g->C = s->C;
     g->pad = s->pad;
     g->sparse_lanes = 8;
+    g->win = s->sigma < s->n ? s->sigma : (s->n > 0 ? s->n : 1);
     g->nslices = s->nslices;
     g->nwords = (int32_t)nwords;
     g->npos = npos;
