@@ -257,6 +257,14 @@ ALBD_HIDDEN int albd_pull(albd_ctx *c, const albd_graph *g, const albd_state *st
                           int32_t level, int64_t items_bound);
 ALBD_HIDDEN int albd_cb_supported(const albd_graph *g);
 
+/* the push kernels of one level (scan, expansion, finalize), queued before
+ * the level's direction is known: they return at once unless the select
+ * kernel chose the push (spec = 1: its learned choice, 0: the heuristic's)
+ * for a nonempty frontier of at most ALBD_SPEC_PUSH_MAX vertices, the bound
+ * below which albd_push_scan takes its one-block scan.  The queue must hold
+ * the frontier (the last level pushed). */
+#define ALBD_SPEC_PUSH_MAX 8192
+ALBD_HIDDEN int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec);
 /* push: queue from the front bitmap (when the last level was a pull) */
 ALBD_HIDDEN int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st);
 /* push: prefix of degrees over the queue (qlen known on the host) */

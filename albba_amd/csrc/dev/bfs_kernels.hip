@@ -30,6 +30,27 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 
+// A kernel queued before its level's direction is known runs only when the
+// select kernel chose it: spec < 0 runs always; otherwise the learned (1) or
+// the heuristic (0) choice must be `choice`, and the frontier nonempty and at
+// most fmax.  The select kernel wrote those counters before this kernel
+// started, and nothing in between changes them, so every thread of every
+// block decides alike.
+struct Spec {
+    int32_t spec, choice;
+    unsigned long long fmax;
+};
+
+__device__ __forceinline__ bool spec_go(const albd_counters *ctr, const Spec &sp)
+{
+    if (sp.spec < 0) return true;
+    const unsigned long long f = ctr->frontier;
+    const int c = sp.spec ? ctr->sel_choice : ctr->heur_choice;
+    return __builtin_amdgcn_readfirstlane(f != 0 && f <= sp.fmax && c == sp.choice ? 1 : 0) != 0;
+}
+
+constexpr Spec kNoSpec{-1, 0, 0ull};
+
 struct PullArgs {
     albd_graph g;
     albd_state st;
@@ -267,13 +288,7 @@ template <int C, int VAR>
 __global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
 {
     extern __shared__ uint32_t lds_sum[];
-    if (a.spec >= 0) {
-        // the same counters for every thread: a uniform early return
-        const albd_counters *ctr = a.st.ctr;
-        const int go = __builtin_amdgcn_readfirstlane(
-            ctr->frontier != 0 && (a.spec ? ctr->sel_choice : ctr->heur_choice) == a.spec_choice ? 1 : 0);
-        if (!go) return;
-    }
+    if (!spec_go(a.st.ctr, Spec{a.spec, a.spec_choice, ~0ull})) return;
     pull_body<C, VAR, pull_block<VAR>()>(a, lds_sum);
 }
 
@@ -640,10 +655,11 @@ __global__ __launch_bounds__(kBlock) void k_select_prep(albd_graph g, albd_state
 // pmin and count the new frontier.  Otherwise the words are st.front.
 template <bool FINALIZE>
 __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_state st,
-                                                           int32_t level)
+                                                           int32_t level, Spec sp)
 {
     __shared__ unsigned int s_cnt[kWavesPerBlock];
     __shared__ unsigned long long s_base;
+    if (!spec_go(st.ctr, sp)) return;
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const uint64_t *words = FINALIZE ? st.next : st.front;
     int32_t *q = FINALIZE ? st.queue_next : st.queue;
@@ -817,9 +833,10 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *tota
     return ex + off;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_small(albd_graph g, albd_state st)
+__global__ __launch_bounds__(kBlock) void k_scan_small(albd_graph g, albd_state st, Spec sp)
 {
     __shared__ int64_t lds[8];
+    if (!spec_go(st.ctr, sp)) return;
     const int64_t qlen = (int64_t)st.ctr->queue_len;
     int64_t carry = 0;
     for (int64_t base = 0; base < qlen; base += kBlock) {
@@ -893,8 +910,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_p3(albd_state st)
 // ---------------------------------------------------------------------------
 // push: expansion over 64-edge chunks
 
-__global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state st)
+__global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state st, Spec sp)
 {
+    if (!spec_go(st.ctr, sp)) return;
     const int lane = lane_id();
     const int64_t qlen = (int64_t)st.ctr->queue_len;
     if (qlen == 0) return;
@@ -1400,7 +1418,7 @@ int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int v
 int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, 0);
+    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, 0, kNoSpec);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1408,8 +1426,8 @@ int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st)
 int albd_push_scan(albd_ctx *c, const albd_graph *g, const albd_state *st, int64_t qlen)
 {
     hipStream_t s = albd_stream(c);
-    if (qlen <= 8192) {
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kBlock), 0, s, *g, *st);
+    if (qlen <= ALBD_SPEC_PUSH_MAX) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kBlock), 0, s, *g, *st, kNoSpec);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -1424,7 +1442,8 @@ int albd_push_scan(albd_ctx *c, const albd_graph *g, const albd_state *st, int64
 
 int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
-    hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, albd_stream(c), *g, *st);
+    hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, albd_stream(c), *g, *st,
+                       kNoSpec);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1487,7 +1506,24 @@ int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
 int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, level);
+    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, level,
+                       kNoSpec);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec)
+{
+    // the frontier bound is the one below which albd_push_scan takes the
+    // one-block scan
+    const Spec sp{spec, ALBBA_CHOICE_PUSH, (unsigned long long)ALBD_SPEC_PUSH_MAX};
+    hipStream_t s = albd_stream(c);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kBlock), 0, s, *g, *st, sp);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, s, *g, *st, sp);
+    HIPCHK(hipGetLastError());
+    const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
+    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st, level, sp);
     HIPCHK(hipGetLastError());
     return 0;
 }

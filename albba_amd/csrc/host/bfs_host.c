@@ -961,13 +961,19 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * after their choice (run_level). */
     const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED;
     unsigned int seq = 0;
-    /* the choice whose pull is already queued for the coming level, or -1 */
-    int queued_pull = -1;
+    /* kernels already queued for the coming level: the choice whose pull is
+     * queued, or -1; whether the push kernels are */
+    int queued_pull = -1, queued_push = 0;
     /* init, then the select kernel turns the root into level 0's frontier */
     if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (spec)
-        rc = select_prep(G, &ls, &sel, &seq) || albd_mail_wait(d, &G->st, seq);
+        rc = select_prep(G, &ls, &sel, &seq) ||
+             /* the root level is a push as a rule; the small-frontier
+              * kernel needs two push levels before it */
+             (!opts.time_levels && (queued_push = 1, albd_push_spec(d, &G->g, &G->st, 0,
+                                                                    policy == ALBBA_SWITCH_LEARNED))) ||
+             albd_mail_wait(d, &G->st, seq);
     else
         rc = albd_select_read(d, &G->g, &G->st, &sel);
     if (rc) return albba_fail(ALBBA_EHIP, "bfs: %s", albd_error());
@@ -1032,7 +1038,9 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         const int small_ok = choice == ALBBA_CHOICE_PUSH && opts.small_frontier_edges > 0 &&
                              (policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED) &&
                              first_edges <= opts.small_frontier_edges && frontier <= push_vertex_cap;
-        const int small = small_ok && small_streak >= 2;
+        /* (never with push kernels queued: they are queued only while
+         * small_streak < 2) */
+        const int small = small_ok && small_streak >= 2 && !queued_push;
         small_streak = small_ok && !small ? small_streak + 1 : 0;
         int nlev = 1;
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
@@ -1040,7 +1048,9 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
             rc = run_small(G, &ls, level, frontier, opts.small_frontier_edges, push_vertex_cap,
                            opts.small_frontier_blocks, opts.time_levels, spec, &nlev);
         else if (spec) {
-            if (choice != queued_pull) rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
+            const int ran = choice == queued_pull ||
+                            (queued_push && choice == ALBBA_CHOICE_PUSH && frontier <= ALBD_SPEC_PUSH_MAX);
+            if (!ran) rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
         }
         else
             rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
@@ -1058,24 +1068,42 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
              * counters */
             commit_pointers(G, &ls, choice);
             if ((rc = select_prep(G, &ls, &sel, &seq))) return albba_fail(rc, "bfs: %s", albd_error());
-            /* After a pull the next level takes the same pull, as a rule:
-             * queue it now, so that the device does not wait for the host to
-             * read the choice.  The kernel reads the choice itself and returns
-             * at once if it differs; the host then queues the chosen kernels
-             * behind it.  Not with per-level events, which must bracket the
-             * chosen kernels. */
+            /* Queue the likely next kernels now, so that the device does
+             * not wait for the host to read the choice: after a pull the
+             * same pull, as a rule; after a push a push while the frontier
+             * cannot outgrow the heuristic's push threshold (it is at most
+             * the edges just pushed), else the pull the policy takes for
+             * large frontiers.  The kernels read the choice themselves and
+             * return at once if it differs; the host then queues the chosen
+             * kernels behind them.  Not with per-level events, which must
+             * bracket the chosen kernels, and no push while the
+             * small-frontier kernel may take the next level. */
             queued_pull = -1;
-            if (choice != ALBBA_CHOICE_PUSH && !opts.time_levels) {
-                const int variant = choice == ALBBA_CHOICE_PULL_SB
+            queued_push = 0;
+            int guess = choice;
+            if (choice == ALBBA_CHOICE_PUSH && !small) {
+                if (first_edges > push_vertex_cap)
+                    guess = policy == ALBBA_SWITCH_LEARNED && albd_cb_supported(&G->g) ? ALBBA_CHOICE_PULL_CB
+                                                                                        : ALBBA_CHOICE_PULL_SB;
+                else if (small_streak >= 2)
+                    guess = -1;
+            }
+            if (guess == ALBBA_CHOICE_PUSH && !opts.time_levels) {
+                if ((rc = albd_push_spec(d, &G->g, &G->st, level + nlev, policy == ALBBA_SWITCH_LEARNED)))
+                    return albba_fail(rc, "bfs: %s", albd_error());
+                queued_push = 1;
+            }
+            if (guess >= 0 && guess != ALBBA_CHOICE_PUSH && !opts.time_levels) {
+                const int variant = guess == ALBBA_CHOICE_PULL_SB
                                         ? ALBD_PULL_LIST
-                                        : (choice == ALBBA_CHOICE_PULL_CB ? ALBD_PULL_LIST_CB : ALBD_PULL_ALL);
+                                        : (guess == ALBBA_CHOICE_PULL_CB ? ALBD_PULL_LIST_CB : ALBD_PULL_ALL);
                 /* lists only shrink: the last length the host knows bounds
                  * the list the preparation builds */
                 if ((rc = albd_pull_spec(d, &G->g, &G->st, variant, level + nlev,
                                          ls.list_valid && ls.list_len > 0 ? ls.list_len : G->g.nwords,
                                          policy == ALBBA_SWITCH_LEARNED)))
                     return albba_fail(rc, "bfs: %s", albd_error());
-                queued_pull = choice;
+                queued_pull = guess;
             }
             if ((rc = albd_mail_wait(d, &G->st, seq))) return albba_fail(rc, "bfs: %s", albd_error());
         } else if ((rc = albd_select_read(d, &G->g, &G->st, &sel))) {
