@@ -391,7 +391,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
     // Every load first (the counter block, the weights, the slots), so that
     // they travel together; then the block sums the slots, and wave 0 alone
     // computes the features, runs the MLP and writes the counter block back
-    // (22 lanes in parallel: thread 0 alone would chain one load per word).
+    // (one lane per word: thread 0 alone would chain one load per word).
     constexpr int kPer = ALBD_NSLOT / kSelectBlock;
     static_assert(ALBD_NSLOT % kSelectBlock == 0, "slots split evenly over the block");
     static_assert(sizeof(albd_slot) == 64, "slot = 3 x 16 counter bytes + padding");
