@@ -126,159 +126,203 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     const uint64_t gmask = SUB == 64 ? ~0ull : (((1ull << SUB) - 1) << ((lane / SUB) * SUB));
 
     unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0, wscan = 0, wbyp = 0;
-    for (;;) {
+    // One word ahead: while a word's columns are scanned, the next word is
+    // claimed, its list entry read, and then its row lengths, visited word
+    // and slice pointer loaded, so that a word starts with its first column
+    // loads rather than two dependent round trips.  A wave holds one word
+    // ahead only, which bounds what it can keep from the block's other waves.
+    auto claim = [&]() -> int64_t {
         int k = 0;
         if (lane == 0) k = atomicAdd(&next_item, 1);
-        k = __shfl(k, 0, 64);
-        const int64_t it = b0 + k;
-        if (it >= b1) break;
+        return b0 + __shfl(k, 0, 64);
+    };
+    auto word_of = [&](int64_t it) -> int32_t {
+        if (it >= b1) return -1;
+        return (VAR == ALBD_PULL_ALL) ? (int32_t)it : a.items[it];
+    };
+    struct Head {
+        int len, len_o;
+        uint64_t vis;
+        int64_t cp;
+    };
+    // row lengths stream like the columns: non-temporal, so that the
+    // frontier and visited bitmaps keep their L2 lines
+    auto load_head = [&](int32_t w) -> Head {
+        Head h{0, 0, ~0ull, 0};
+        if (w < 0) return h;
+        const int64_t p = (int64_t)w * 64 + lane;
+        if constexpr (C <= 64) {
+            h.len = __builtin_nontemporal_load(&g.row_len[p]);
+            h.cp = g.slice_ptr[p / C];
+        } else {
+            const int64_t s = w >> 1;
+            h.len = __builtin_nontemporal_load(&g.row_len[p]);
+            h.len_o = __builtin_nontemporal_load(&g.row_len[s * 128 + (((w & 1) ^ 1) << 6) + lane]);
+            h.cp = g.slice_ptr[s];
+        }
+        h.vis = a.st.visited[w];
+        return h;
+    };
+    int32_t w = word_of(claim());
+    Head h = load_head(w);
+    while (w >= 0) {
+        const int32_t w1 = word_of(claim());
+        Head h1{0, 0, ~0ull, 0};
+        bool ahead = false; // h1 loaded (wave-uniform)
         {
-            const int32_t w = (VAR == ALBD_PULL_ALL) ? (int32_t)it : a.items[it];
             const int64_t p = (int64_t)w * 64 + lane;
             // lane row lengths replace per-column active-lane counts: rows of
             // a slice are sorted by length, so column j holds the lanes whose
             // row is longer than j and its size is a ballot away
-            int s, l, len, len_o = 0;
-            // row lengths stream like the columns: non-temporal, so that the
-            // frontier and visited bitmaps keep their L2 lines
-            if constexpr (C <= 64) {
-                s = (int)(p / C);
+            int l;
+            if constexpr (C <= 64)
                 l = (int)(p % C);
-                len = __builtin_nontemporal_load(&g.row_len[p]);
-            } else {
-                s = w >> 1;
+            else
                 l = ((w & 1) << 6) + lane;
-                len = __builtin_nontemporal_load(&g.row_len[p]);
-                len_o = __builtin_nontemporal_load(&g.row_len[(int64_t)s * 128 + (((w & 1) ^ 1) << 6) + lane]);
-            }
-            const uint64_t vis = a.st.visited[w];
-            int64_t cp = g.slice_ptr[s];
+            const int len = h.len, len_o = h.len_o;
+            const uint64_t vis = h.vis;
+            int64_t cp = h.cp;
             uint64_t unres = __ballot(len > 0) & ~vis;
             if (unres == 0) {
                 ++wbyp; // slice bypass: every lane with a row is visited
-                continue;
-            }
-            ++wscan;
-            const uint64_t start = unres;
-            int parent_pos = -1;
-            int j = 0, step = 1;
-            // columns are loaded 1, 2, then 4 at a time: most lanes of a dense
-            // level hit in the first column, the survivors tend to run long
-            for (;;) {
-                const bool mine = (unres >> lane) & 1ull;
-                const uint64_t live = __ballot(mine && j < len);
-                // few live lanes left: their rows go to the row phase below
-                if (__popcll(live) <= g.sparse_lanes) break;
-                int cidx[4];
-                bool av[4], hit[4];
+            } else {
+                ++wscan;
+                const uint64_t start = unres;
+                int parent_pos = -1;
+                int j = 0, step = 1;
+                // columns are loaded 1, 2, then 4 at a time: most lanes of a
+                // dense level hit in the first column, the survivors tend to
+                // run long
+                for (;;) {
+                    const bool mine = (unres >> lane) & 1ull;
+                    const uint64_t live = __ballot(mine && j < len);
+                    // few live lanes left: their rows go to the row phase below
+                    if (__popcll(live) <= g.sparse_lanes) break;
+                    int cidx[4];
+                    bool av[4], hit[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    av[u] = false;
-                    cidx[u] = 0;
-                    if (u < step) {
-                        const int col = j + u;
-                        int act;
-                        if constexpr (C <= 64)
-                            act = __popcll(__ballot(col < len) & gmask);
-                        else
-                            act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
-                        av[u] = mine && col < len;
-                        // streaming: non-temporal, so that the frontier and
-                        // visited bitmaps stay in L2
-                        if (av[u]) cidx[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
-                        cp += (act + padm) & ~padm;
+                    for (int u = 0; u < 4; ++u) {
+                        av[u] = false;
+                        cidx[u] = 0;
+                        if (u < step) {
+                            const int col = j + u;
+                            int act;
+                            if constexpr (C <= 64)
+                                act = __popcll(__ballot(col < len) & gmask);
+                            else
+                                act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
+                            av[u] = mine && col < len;
+                            // streaming: non-temporal, so that the frontier
+                            // and visited bitmaps stay in L2
+                            if (av[u]) cidx[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
+                            cp += (act + padm) & ~padm;
+                        }
                     }
-                }
+                    // the next word's head travels with this word's first
+                    // columns
+                    if (!ahead) {
+                        h1 = load_head(w1);
+                        ahead = true;
+                    }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    bool h = false;
-                    if (av[u]) {
-                        const int c = cidx[u];
-                        const int wd = c >> 6;
-                        bool maybe = true;
-                        if constexpr (VAR == ALBD_PULL_LIST_CB)
-                            maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
-                        if (maybe) h = (a.st.front[wd] >> (c & 63)) & 1ull;
+                    for (int u = 0; u < 4; ++u) {
+                        bool hh = false;
+                        if (av[u]) {
+                            const int c = cidx[u];
+                            const int wd = c >> 6;
+                            bool maybe = true;
+                            if constexpr (VAR == ALBD_PULL_LIST_CB)
+                                maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                            if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
+                        }
+                        hit[u] = hh;
                     }
-                    hit[u] = h;
-                }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (u < step) {
-                        const bool h = hit[u] && ((unres >> lane) & 1ull);
-                        if (h) parent_pos = cidx[u];
-                        unres &= ~__ballot(h);
-                        scanned += __popcll(__ballot(av[u]));
+                    for (int u = 0; u < 4; ++u) {
+                        if (u < step) {
+                            const bool hh = hit[u] && ((unres >> lane) & 1ull);
+                            if (hh) parent_pos = cidx[u];
+                            unres &= ~__ballot(hh);
+                            scanned += __popcll(__ballot(av[u]));
+                        }
+                    }
+                    j += step;
+                    if (unres == 0) break;
+                    step = step < 4 ? 2 * step : 4;
+                }
+                if (!ahead) {
+                    h1 = load_head(w1);
+                    ahead = true;
+                }
+                // Row phase: the r <= sparse_lanes remaining rows (columns j..
+                // onward) are scanned from the row-major copy side by side,
+                // each by a group of G = 64/2^ceil(log2 r) lanes reading G
+                // consecutive entries per step; the first hit in row order is
+                // the min id.
+                const uint64_t rem = __ballot(((unres >> lane) & 1ull) && j < len);
+                const int r = __popcll(rem);
+                if (r) {
+                    int G = 64;
+                    while (G * r > 64) G >>= 1;
+                    const int gid = lane / G, gl = lane % G;
+                    const uint64_t gmask_r = (G == 64) ? ~0ull : (((1ull << G) - 1) << (gid * G));
+                    int L = -1; // the gid-th remaining lane
+                    {
+                        uint64_t t = rem;
+                        for (int k = 0; k < gid && t; ++k) t &= t - 1;
+                        if (gid < r) L = __ffsll((unsigned long long)t) - 1;
+                    }
+                    const int rl = __shfl(len, L < 0 ? 0 : L, 64);
+                    const int64_t rs = L >= 0 ? g.pull_ptr[(int64_t)w * 64 + L] : 0;
+                    int got = -1;
+                    for (int k = j;; k += G) {
+                        const bool live_g = L >= 0 && got < 0 && k < rl;
+                        if (__ballot(live_g) == 0) break;
+                        const int e = k + gl;
+                        bool hh = false;
+                        int c = 0;
+                        if (live_g && e < rl) {
+                            c = g.pull_col[rs + e];
+                            const int wd = c >> 6;
+                            bool maybe = true;
+                            if constexpr (VAR == ALBD_PULL_LIST_CB)
+                                maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                            if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
+                        }
+                        scanned += __popcll(__ballot(live_g && e < rl));
+                        const uint64_t hm = __ballot(hh) & gmask_r;
+                        const int src = hm ? __ffsll((unsigned long long)hm) - 1 : lane;
+                        const int first = __shfl(c, src, 64);
+                        if (hm && got < 0) got = first;
+                    }
+                    for (int q = 0; q < r; ++q) {
+                        const int Lq = __shfl(L, q * G, 64);
+                        const int gq = __shfl(got, q * G, 64);
+                        if (gq >= 0) {
+                            unres &= ~(1ull << Lq);
+                            if (lane == Lq) parent_pos = gq;
+                        }
                     }
                 }
-                j += step;
-                if (unres == 0) break;
-                step = step < 4 ? 2 * step : 4;
-            }
-            // Row phase: the r <= sparse_lanes remaining rows (columns j..
-            // onward) are scanned from the row-major copy side by side, each
-            // by a group of G = 64/2^ceil(log2 r) lanes reading G consecutive
-            // entries per step; the first hit in row order is the min id.
-            const uint64_t rem = __ballot(((unres >> lane) & 1ull) && j < len);
-            const int r = __popcll(rem);
-            if (r) {
-                int G = 64;
-                while (G * r > 64) G >>= 1;
-                const int gid = lane / G, gl = lane % G;
-                const uint64_t gmask_r = (G == 64) ? ~0ull : (((1ull << G) - 1) << (gid * G));
-                int L = -1; // the gid-th remaining lane
-                {
-                    uint64_t t = rem;
-                    for (int k = 0; k < gid && t; ++k) t &= t - 1;
-                    if (gid < r) L = __ffsll((unsigned long long)t) - 1;
-                }
-                const int rl = __shfl(len, L < 0 ? 0 : L, 64);
-                const int64_t rs = L >= 0 ? g.pull_ptr[(int64_t)w * 64 + L] : 0;
-                int got = -1;
-                for (int k = j;; k += G) {
-                    const bool live_g = L >= 0 && got < 0 && k < rl;
-                    if (__ballot(live_g) == 0) break;
-                    const int e = k + gl;
-                    bool h = false;
-                    int c = 0;
-                    if (live_g && e < rl) {
-                        c = g.pull_col[rs + e];
-                        const int wd = c >> 6;
-                        bool maybe = true;
-                        if constexpr (VAR == ALBD_PULL_LIST_CB)
-                            maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
-                        if (maybe) h = (a.st.front[wd] >> (c & 63)) & 1ull;
+                const uint64_t found = start & ~unres;
+                if (found) {
+                    if (lane == 0) {
+                        a.st.next[w] = found;
+                        a.st.visited[w] = vis | found;
                     }
-                    scanned += __popcll(__ballot(live_g && e < rl));
-                    const uint64_t hm = __ballot(h) & gmask_r;
-                    const int src = hm ? __ffsll((unsigned long long)hm) - 1 : lane;
-                    const int first = __shfl(c, src, 64);
-                    if (hm && got < 0) got = first;
-                }
-                for (int q = 0; q < r; ++q) {
-                    const int Lq = __shfl(L, q * G, 64);
-                    const int gq = __shfl(got, q * G, 64);
-                    if (gq >= 0) {
-                        unres &= ~(1ull << Lq);
-                        if (lane == Lq) parent_pos = gq;
+                    if ((found >> lane) & 1ull) {
+                        // position order: the lanes of a word write one run
+                        a.st.lp[p] = pack_lp(a.level + 1, g.row_order[parent_pos]);
+                        edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
                     }
+                    cnt += __popcll(found);
+                    nzw += 1;
                 }
-            }
-            const uint64_t found = start & ~unres;
-            if (found) {
-                if (lane == 0) {
-                    a.st.next[w] = found;
-                    a.st.visited[w] = vis | found;
-                }
-                if ((found >> lane) & 1ull) {
-                    // position order: the lanes of a word write one run
-                    a.st.lp[p] = pack_lp(a.level + 1, g.row_order[parent_pos]);
-                    edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
-                }
-                cnt += __popcll(found);
-                nzw += 1;
             }
         }
+        if (!ahead) h1 = load_head(w1);
+        w = w1;
+        h = h1;
     }
     edges = wave_sum(edges);
     if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp);
