@@ -1500,7 +1500,7 @@ int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int3
     if (max_levels < 1) max_levels = 1;
     if (albd_coop_off(c)) return ALBD_NOCOOP;
     // every workgroup of the grid barrier must be resident at once
-    static int per_cu = -1; // the same for every device of this build
+    static int per_cu = -1; // one gfx950 code object: the same on every device
     if (per_cu < 0)
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
     const int cap = per_cu * albd_num_cus(c);

@@ -435,8 +435,8 @@ def main():
                     help="run one policy only (profiling); 'both' checks bit-exactness")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a single BFS would shard as a 1D slice partition, see DESIGN.md)",
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a single BFS would shard as a 1D partition of the SELL slices)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":

@@ -85,7 +85,7 @@ def test_read_graph_is_the_transposed_working_matrix(name):
     np.testing.assert_array_equal(g.col, GM[f"{name}__csc_idx"])
 
 
-@pytest.mark.skipif(not O.ref_available(), reason="reference build absent (GPU box)")
+@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
 @pytest.mark.parametrize("name", MTX)
 def test_read_Dmatrix_matches_live_reference(name):
     ref = O.ref_read(os.path.join(MTXDIR, name + ".mtx"))

@@ -67,7 +67,7 @@ def test_oracle_sell_matches_reference_fixture(name, C_, sigma):
         np.testing.assert_array_equal(got[k], GG[f"{name}__sell_{C_}_{sigma}__{k}"], err_msg=k)
 
 
-@pytest.mark.skipif(not O.ref_available(), reason="reference build absent (GPU box)")
+@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
 def test_oracle_sell_matches_live_reference():
     import albba_amd as A
 
