@@ -1026,10 +1026,12 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
 // atomicMin(key[v], L << 32 | id(u)).  The old value tells the discoverer
 // (all ones: first at this level), and the key keeps the smallest (level,
 // parent id) pair, i.e. the min-id parent at v's BFS level: keys of earlier
-// levels are smaller and never change.  The visited bitmap is only a filter
-// that saves atomics (a stale copy costs an atomic, never a wrong answer),
-// so level L's vertices are written out and merged into visited during
-// level L+1, and the keys go back to all ones when the kernel ends.
+// levels are smaller and never change.  Two filters save atomics: the
+// visited bitmap as it was when the kernel started, and the key itself when
+// it already holds an earlier level (keys only decrease, so a stale copy
+// costs an atomic, never a wrong answer).  Neither is written during the
+// launch: the vertices it finds get their level/parent words, visited bits
+// and keys back to all ones when it ends.
 //
 // Cross-workgroup data (found list, queues, keys, counters) moves through
 // device-coherent loads, stores and atomics; every wave waits for its own
@@ -1079,19 +1081,12 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
         s_sum = 0;
     }
     __syncthreads();
-    int64_t fo_prev = 0, fo = 0; // found[fo_prev, fo): the previous level
+    int64_t fo = 0; // found[0, fo): the earlier levels of this launch
     bool ok = true;
     for (int k = 0; ok; ++k) {
         const uint32_t lvl = (uint32_t)(a.level0 + k + 1);
         int64_t *cnt = st.level_log + 4 * k; // found, edges, queue entries of level k
-        // (1) the previous level: level, parent, visited
-        for (int64_t i = fo_prev + gtid; i < fo; i += gthreads) {
-            const int32_t v = ld_coherent(&found[i]);
-            const unsigned long long key = ld_coherent(&st.pkey[v]);
-            st.lp[v] = key; // the same (level, parent) packing
-            atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), 1ull << (v & 63));
-        }
-        // (2) expansion: workgroup w takes queue entries [q0, q1)
+        // expansion: workgroup w takes queue entries [q0, q1)
         const int64_t per = (qlen + G - 1) / G;
         const int64_t q0 = (int64_t)w * per, q1 = q0 + per < qlen ? q0 + per : qlen;
         unsigned long long my_edges = 0;
@@ -1139,13 +1134,19 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                     }
                 }
                 uint64_t vis[kSmallPer];
+                unsigned long long pk[kSmallPer];
 #pragma unroll
-                for (int i = 0; i < kSmallPer; ++i) vis[i] = okv[i] ? st.visited[v[i] >> 6] : ~0ull;
+                for (int i = 0; i < kSmallPer; ++i) {
+                    vis[i] = okv[i] ? st.visited[v[i] >> 6] : ~0ull;
+                    pk[i] = okv[i] ? st.pkey[v[i]] : 0ull;
+                }
                 bool won[kSmallPer];
 #pragma unroll
                 for (int i = 0; i < kSmallPer; ++i) {
                     won[i] = false;
-                    if (!((vis[i] >> (v[i] & 63)) & 1ull)) {
+                    // unvisited at the launch and not found at an earlier
+                    // level of it
+                    if (!((vis[i] >> (v[i] & 63)) & 1ull) && (uint32_t)(pk[i] >> 32) >= lvl) {
                         const unsigned long long key =
                             ((unsigned long long)lvl << 32) | (unsigned long long)ou[i];
                         const unsigned long long old = atomicMin(&st.pkey[v[i]], key);
@@ -1219,7 +1220,13 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                 my_words += old == 0 ? 1 : 0;
                 st.pkey[v] = ~0ull;
             }
-            for (int64_t i = gtid; i < fo; i += gthreads) st.pkey[ld_coherent(&found[i])] = ~0ull;
+            // the earlier levels' vertices: level, parent and visited too
+            for (int64_t i = gtid; i < fo; i += gthreads) {
+                const int32_t v = ld_coherent(&found[i]);
+                st.lp[v] = ld_coherent(&st.pkey[v]); // the same (level, parent) packing
+                atomicOr(reinterpret_cast<unsigned long long *>(&st.visited[v >> 6]), 1ull << (v & 63));
+                st.pkey[v] = ~0ull;
+            }
             my_words = wave_sum(my_words);
             if (lane == 0 && my_words) atomicAdd(&st.slots[1 + (w % (ALBD_NSLOT - 1))].next_words, my_words);
             if (w == 0 && tid == 0) {
@@ -1237,7 +1244,6 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
             st.ctr->reached += (unsigned long long)na;
             st.ctr->reached_edges += (unsigned long long)edges;
         }
-        fo_prev = fo;
         fo = total;
         int32_t *t = qa;
         qa = qb;
