@@ -1135,10 +1135,15 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                 }
                 uint64_t vis[kSmallPer];
                 unsigned long long pk[kSmallPer];
+                int64_t d0[kSmallPer], d1[kSmallPer];
+                // the filters and the out-degree together: one round trip
+                // ahead of the atomic instead of one after it
 #pragma unroll
                 for (int i = 0; i < kSmallPer; ++i) {
                     vis[i] = okv[i] ? st.visited[v[i] >> 6] : ~0ull;
                     pk[i] = okv[i] ? st.pkey[v[i]] : 0ull;
+                    d0[i] = okv[i] ? g.push_ptr[v[i]] : 0;
+                    d1[i] = okv[i] ? g.push_ptr[v[i] + 1] : 0;
                 }
                 bool won[kSmallPer];
 #pragma unroll
@@ -1156,7 +1161,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                 int64_t dv[kSmallPer];
 #pragma unroll
                 for (int i = 0; i < kSmallPer; ++i)
-                    dv[i] = won[i] ? g.push_ptr[v[i] + 1] - g.push_ptr[v[i]] : 0;
+                    dv[i] = won[i] ? d1[i] - d0[i] : 0;
 #pragma unroll
                 for (int i = 0; i < kSmallPer; ++i) {
                     const unsigned int pf = wave_append(&s_nf, won[i]);
