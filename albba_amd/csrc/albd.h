@@ -19,7 +19,7 @@ extern "C" {
 /* Counter slots: blocks spread their per-level atomics over this many 64-byte
  * slots (same-address device atomics serialise at roughly 100 ns each), and
  * the select kernel sums them at the end of the level. */
-#define ALBD_NSLOT 1024
+#define ALBD_NSLOT 256
 
 typedef struct albd_ctx albd_ctx;
 
