@@ -262,18 +262,22 @@ ALBD_HIDDEN int albd_cb_supported(const albd_graph *g);
  * kernel chose the push (spec = 1: its learned choice, 0: the heuristic's)
  * for a nonempty frontier of at most ALBD_SPEC_PUSH_MAX vertices, the bound
  * below which albd_push_scan takes its one-block scan.  The queue must hold
- * the frontier (the last level pushed). */
+ * the frontier (the last level pushed); the finalize lists the next queue as
+ * albd_push_finalize does with build_max. */
 #define ALBD_SPEC_PUSH_MAX 8192
-ALBD_HIDDEN int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec);
+ALBD_HIDDEN int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec,
+                               int64_t build_max);
 /* push: queue from the front bitmap (when the last level was a pull) */
 ALBD_HIDDEN int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st);
 /* push: prefix of degrees over the queue (qlen known on the host) */
 ALBD_HIDDEN int albd_push_scan(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                int64_t qlen);
 ALBD_HIDDEN int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_state *st);
-/* push: merge next into visited, write level/parent from pmin, build queue_next */
+/* push: merge next into visited, write level/parent from pmin, and list
+ * queue_next (ctr->qnext_len) while the level's frontier edges
+ * (ctr->frontier_edges) are at most build_max; else it stays empty */
 ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st,
-                                   int32_t level);
+                                   int32_t level, int64_t build_max);
 /* Small-frontier push levels in one cooperative kernel of `blocks`
  * workgroups (0: the default; capped at what can be co-resident): starting from the
  * queue (st.queue, ctr->queue_len), run push levels level0, level0+1, ...

@@ -696,14 +696,18 @@ __global__ __launch_bounds__(kBlock) void k_select_prep(albd_graph g, albd_state
 
 // Words -> queue of positions with out-degree > 0.  FINALIZE (after a push):
 // the words are st.next; merge them into visited, write level/parent from
-// pmin and count the new frontier.  Otherwise the words are st.front.
+// pmin and count the new frontier; the queue only while the level's
+// frontier edges (the select kernel's count) are at most build_max: a larger
+// push likely leads to a pull, which needs none, and a push after it lists
+// the bitmap.  Otherwise the words are st.front.
 template <bool FINALIZE>
 __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_state st,
-                                                           int32_t level, Spec sp)
+                                                           int32_t level, Spec sp, long long build_max)
 {
     __shared__ unsigned int s_cnt[kWavesPerBlock];
     __shared__ unsigned long long s_base;
     if (!spec_go(st.ctr, sp)) return;
+    const bool build = !FINALIZE || (long long)st.ctr->frontier_edges <= build_max;
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const uint64_t *words = FINALIZE ? st.next : st.front;
     int32_t *q = FINALIZE ? st.queue_next : st.queue;
@@ -742,6 +746,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
             const uint64_t km = __ballot(keep);
             if (lane == k) keepw = km;
         }
+        if (!build) continue; // kernel-uniform
         unsigned long long total;
         const unsigned long long excl =
             wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(keepw), &total);
@@ -1473,7 +1478,8 @@ int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int v
 int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, 0, kNoSpec);
+    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, 0, kNoSpec,
+                       LLONG_MAX);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1558,16 +1564,17 @@ int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
     return albd_mail_wait(c, st, seq);
 }
 
-int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level)
+int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int64_t build_max)
 {
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
     hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, level,
-                       kNoSpec);
+                       kNoSpec, (long long)build_max);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec)
+int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int spec,
+                   int64_t build_max)
 {
     // the frontier bound is the one below which albd_push_scan takes the
     // one-block scan
@@ -1578,7 +1585,8 @@ int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32
     hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, s, *g, *st, sp);
     HIPCHK(hipGetLastError());
     const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st, level, sp);
+    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st, level, sp,
+                       (long long)build_max);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -754,7 +754,7 @@ static int run_level(albba_graph *G, const loop_state *ls, int choice, int32_t l
         if (mark && (rc = albd_event_record(d, 6))) return rc;
         if ((rc = albd_push_expand(d, &G->g, &G->st))) return rc;
         if (mark && (rc = albd_event_record(d, 7))) return rc;
-        return albd_push_finalize(d, &G->g, &G->st, level);
+        return albd_push_finalize(d, &G->g, &G->st, level, INT64_MAX);
     }
     int variant = ALBD_PULL_ALL;
     if (choice == ALBBA_CHOICE_PULL_SB) variant = ALBD_PULL_LIST;
@@ -788,9 +788,11 @@ static int select_prep(albba_graph *G, const loop_state *ls, const albd_select_a
     return albd_select_prep_post(G->ctx->d, &G->g, &G->st, sel, seq, ls->list_valid, albd_cb_supported(&G->g));
 }
 
-/* The kernels of one level after prep_level (no pointer swaps). */
+/* The kernels of one level after prep_level (no pointer swaps); a push
+ * lists the next frontier's queue while its frontier edges are at most
+ * build_max. */
 static int run_main(albba_graph *G, const loop_state *ls, int choice, int32_t level,
-                    int64_t frontier, int mark)
+                    int64_t frontier, int mark, int64_t build_max)
 {
     albd_ctx *d = G->ctx->d;
     int rc;
@@ -800,7 +802,7 @@ static int run_main(albba_graph *G, const loop_state *ls, int choice, int32_t le
         if (mark && (rc = albd_event_record(d, 6))) return rc;
         if ((rc = albd_push_expand(d, &G->g, &G->st))) return rc;
         if (mark && (rc = albd_event_record(d, 7))) return rc;
-        return albd_push_finalize(d, &G->g, &G->st, level);
+        return albd_push_finalize(d, &G->g, &G->st, level, build_max);
     }
     int variant = ALBD_PULL_ALL;
     if (choice == ALBBA_CHOICE_PULL_SB) variant = ALBD_PULL_LIST;
@@ -834,7 +836,7 @@ static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_
         if ((rc = albd_push_scan(d, &G->g, &G->st, frontier)) ||
             (rc = albd_push_expand(d, &G->g, &G->st)) || (mark && (rc = albd_event_record(d, 7))))
             return rc;
-        return albd_push_finalize(d, &G->g, &G->st, level);
+        return albd_push_finalize(d, &G->g, &G->st, level, INT64_MAX);
     }
     if (rc) return rc;
     if (mark && (rc = albd_event_record(d, 7))) return rc;
@@ -858,8 +860,9 @@ static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_
 }
 
 /* Pointer swaps and bookkeeping after a committed level (list length from
- * the level's counters: commit_counts). */
-static void commit_pointers(albba_graph *G, loop_state *ls, int choice)
+ * the level's counters: commit_counts); queue_built: a pushed level listed
+ * the next frontier's queue. */
+static void commit_pointers(albba_graph *G, loop_state *ls, int choice, int queue_built)
 {
     uint64_t *t = G->st.front;
     G->st.front = G->st.next;
@@ -872,7 +875,7 @@ static void commit_pointers(albba_graph *G, loop_state *ls, int choice)
         int32_t *q = G->st.queue;
         G->st.queue = G->st.queue_next;
         G->st.queue_next = q;
-        ls->have_queue = 1;
+        ls->have_queue = queue_built;
         ls->prev_push = 1;
     } else {
         ls->have_queue = 0;
@@ -952,6 +955,13 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
 
     /* the heuristic pushes while F <= (int)(n * 0.04) */
     const int64_t push_vertex_cap = (int64_t)(int)((double)G->n * 0.04);
+    /* A push level lists the next frontier's queue only while its frontier
+     * edges (which bound the next frontier) stay at most queue_max: past it
+     * the next level is likely a pull, which needs no queue (a push after it
+     * lists the bitmap instead).  The heuristic pushes exactly while the
+     * frontier is at most push_vertex_cap; the learned selector switches to
+     * pulls about an order of magnitude earlier. */
+    const int64_t queue_max = policy == ALBBA_SWITCH_LEARNED ? push_vertex_cap / 8 : push_vertex_cap;
     loop_state ls = {1, 0, 0, 0, 0};
     int small_streak = 0; /* consecutive plain push levels that stayed small */
     int32_t level = 0;
@@ -972,7 +982,8 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
              /* the root level is a push as a rule; the small-frontier
               * kernel needs two push levels before it */
              (!opts.time_levels && (queued_push = 1, albd_push_spec(d, &G->g, &G->st, 0,
-                                                                    policy == ALBBA_SWITCH_LEARNED))) ||
+                                                                    policy == ALBBA_SWITCH_LEARNED,
+                                                                    queue_max))) ||
              albd_mail_wait(d, &G->st, seq);
     else
         rc = albd_select_read(d, &G->g, &G->st, &sel);
@@ -1014,7 +1025,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                     if ((rc = albd_d2d(d, G->st.visited, G->scratch_visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
                         (rc = albd_d2d(d, G->st.ctr, G->scratch_ctr, sizeof(albd_counters))) ||
                         (rc = albd_level_begin_flags(d, &G->g, &G->st, LB_SLOTS)) || (rc = prep_level(G, &ls, level)) ||
-                        (rc = albd_event_record(d, 4)) || (rc = run_main(G, &ls, c, level, frontier, 0)) ||
+                        (rc = albd_event_record(d, 4)) || (rc = run_main(G, &ls, c, level, frontier, 0, INT64_MAX)) ||
                         (rc = albd_event_record(d, 5)) || (rc = albd_event_elapsed(d, 4, 5, &t[r])))
                         return albba_fail(rc, "bfs: %s", albd_error());
                 }
@@ -1043,6 +1054,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         const int small = small_ok && small_streak >= 2 && !queued_push;
         small_streak = small_ok && !small ? small_streak + 1 : 0;
         int nlev = 1;
+        int queue_built = 1;
         if (opts.time_levels && (rc = albd_event_record(d, 2))) return albba_fail(rc, "bfs: %s", albd_error());
         if (small)
             rc = run_small(G, &ls, level, frontier, opts.small_frontier_edges, push_vertex_cap,
@@ -1050,7 +1062,10 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         else if (spec) {
             const int ran = choice == queued_pull ||
                             (queued_push && choice == ALBBA_CHOICE_PUSH && frontier <= ALBD_SPEC_PUSH_MAX);
-            if (!ran) rc = run_main(G, &ls, choice, level, frontier, opts.time_levels);
+            /* the finalize kernel applies queue_max to the select kernel's
+             * edge count, queued ahead or not */
+            queue_built = choice != ALBBA_CHOICE_PUSH || first_edges <= queue_max;
+            if (!ran) rc = run_main(G, &ls, choice, level, frontier, opts.time_levels, queue_max);
         }
         else
             rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
@@ -1066,13 +1081,12 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
             /* swap to the next level's state, post the select kernel with
              * the next level's preparation beside it, then wait for the
              * counters */
-            commit_pointers(G, &ls, choice);
+            commit_pointers(G, &ls, choice, queue_built);
             if ((rc = select_prep(G, &ls, &sel, &seq))) return albba_fail(rc, "bfs: %s", albd_error());
             /* Queue the likely next kernels now, so that the device does
              * not wait for the host to read the choice: after a pull the
-             * same pull, as a rule; after a push a push while the frontier
-             * cannot outgrow the heuristic's push threshold (it is at most
-             * the edges just pushed), else the pull the policy takes for
+             * same pull, as a rule; after a push a push while it listed the
+             * next queue (queue_max), else the pull the policy takes for
              * large frontiers.  The kernels read the choice themselves and
              * return at once if it differs; the host then queues the chosen
              * kernels behind them.  Not with per-level events, which must
@@ -1082,14 +1096,15 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
             queued_push = 0;
             int guess = choice;
             if (choice == ALBBA_CHOICE_PUSH && !small) {
-                if (first_edges > push_vertex_cap)
+                if (first_edges > queue_max)
                     guess = policy == ALBBA_SWITCH_LEARNED && albd_cb_supported(&G->g) ? ALBBA_CHOICE_PULL_CB
                                                                                         : ALBBA_CHOICE_PULL_SB;
                 else if (small_streak >= 2)
                     guess = -1;
             }
             if (guess == ALBBA_CHOICE_PUSH && !opts.time_levels) {
-                if ((rc = albd_push_spec(d, &G->g, &G->st, level + nlev, policy == ALBBA_SWITCH_LEARNED)))
+                if ((rc = albd_push_spec(d, &G->g, &G->st, level + nlev, policy == ALBBA_SWITCH_LEARNED,
+                                         queue_max)))
                     return albba_fail(rc, "bfs: %s", albd_error());
                 queued_push = 1;
             }
@@ -1143,7 +1158,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                 if (stats->bypassed) stats->bypassed[lv] = push ? 0 : sums.bypassed;
             }
         }
-        if (!spec) commit_pointers(G, &ls, choice);
+        if (!spec) commit_pointers(G, &ls, choice, 1);
         commit_counts(&ls, choice, &sums);
         frontier = (int64_t)hc->frontier;
         level += nlev;
