@@ -344,6 +344,7 @@ __global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
 // wave's count (lane 0's value counts); returns the wave's first slot.  Same
 // address device atomics serialise, so the word passes take one per block
 // step rather than one per wave.
+template <int NW = kWavesPerBlock>
 __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *counter,
                                                            unsigned int wave_count,
                                                            unsigned int *s_cnt,
@@ -354,7 +355,7 @@ __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned int tot = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) tot += s_cnt[i];
+        for (int i = 0; i < NW; ++i) tot += s_cnt[i];
         *s_base = tot ? atomicAdd(counter, (unsigned long long)tot) : 0ull;
     }
     __syncthreads();
@@ -700,11 +701,18 @@ __global__ __launch_bounds__(kBlock) void k_select_prep(albd_graph g, albd_state
 // frontier edges (the select kernel's count) are at most build_max: a larger
 // push likely leads to a pull, which needs none, and a push after it lists
 // the bitmap.  Otherwise the words are st.front.
+//
+// 1024-thread blocks: the list reservation is one same-address atomic per
+// block, and those serialise (256-thread blocks made the reservations the
+// larger part of a big push level's finalize).
+constexpr int kQBlock = 1024;
+constexpr int kQWaves = kQBlock / 64;
+
 template <bool FINALIZE>
-__global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_state st,
-                                                           int32_t level, Spec sp, long long build_max)
+__global__ __launch_bounds__(kQBlock) void k_words_to_queue(albd_graph g, albd_state st,
+                                                            int32_t level, Spec sp, long long build_max)
 {
-    __shared__ unsigned int s_cnt[kWavesPerBlock];
+    __shared__ unsigned int s_cnt[kQWaves];
     __shared__ unsigned long long s_base;
     if (!spec_go(st.ctr, sp)) return;
     const bool build = !FINALIZE || (long long)st.ctr->frontier_edges <= build_max;
@@ -715,8 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
     const int64_t ngroups = ((int64_t)g.nwords + 63) >> 6;
     unsigned long long cnt = 0, nzw = 0, edges = 0;
     // block-uniform steps of one 64-word group per wave
-    for (int64_t gb = (int64_t)blockIdx.x * kWavesPerBlock; gb < ngroups;
-         gb += (int64_t)gridDim.x * kWavesPerBlock) {
+    for (int64_t gb = (int64_t)blockIdx.x * kQWaves; gb < ngroups; gb += (int64_t)gridDim.x * kQWaves) {
         const int64_t grp = gb + wid;
         const int64_t w = grp * 64 + lane;
         const uint64_t word = (grp < ngroups && w < g.nwords) ? words[w] : 0ull;
@@ -750,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_words_to_queue(albd_graph g, albd_st
         unsigned long long total;
         const unsigned long long excl =
             wave_exclusive_scan<unsigned long long>((unsigned long long)__popcll(keepw), &total);
-        const unsigned long long base = block_reserve(qlen, (unsigned int)total, s_cnt, &s_base);
+        const unsigned long long base = block_reserve<kQWaves>(qlen, (unsigned int)total, s_cnt, &s_base);
         uint64_t km = __ballot(keepw != 0);
         while (km) {
             const int k = __ffsll((unsigned long long)km) - 1;
@@ -1269,6 +1276,15 @@ int grid_persistent(albd_ctx *c, int blocks_per_cu)
     return g < 8 ? 8 : g;
 }
 
+// blocks of k_words_to_queue (1024 threads) for `groups` 64-word groups
+int grid_for_qgroups(albd_ctx *c, int64_t groups)
+{
+    int64_t blocks = (groups + kQWaves - 1) / kQWaves;
+    int64_t cap = (int64_t)albd_num_cus(c) * 2;
+    if (blocks > cap) blocks = cap;
+    return blocks < 1 ? 1 : (int)blocks;
+}
+
 int grid_for_groups(albd_ctx *c, int64_t groups)
 {
     int64_t blocks = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -1477,8 +1493,8 @@ int albd_pull_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int v
 
 int albd_front_to_queue(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
-    const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, 0, kNoSpec,
+    const int grid = grid_for_qgroups(c, ((int64_t)g->nwords + 63) / 64);
+    hipLaunchKernelGGL(k_words_to_queue<false>, dim3(grid), dim3(kQBlock), 0, albd_stream(c), *g, *st, 0, kNoSpec,
                        LLONG_MAX);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1566,8 +1582,8 @@ int albd_select_read(albd_ctx *c, const albd_graph *g, const albd_state *st,
 
 int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_t level, int64_t build_max)
 {
-    const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, albd_stream(c), *g, *st, level,
+    const int grid = grid_for_qgroups(c, ((int64_t)g->nwords + 63) / 64);
+    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kQBlock), 0, albd_stream(c), *g, *st, level,
                        kNoSpec, (long long)build_max);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1584,8 +1600,8 @@ int albd_push_spec(albd_ctx *c, const albd_graph *g, const albd_state *st, int32
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_push_expand, dim3(grid_persistent(c, 8)), dim3(kBlock), 0, s, *g, *st, sp);
     HIPCHK(hipGetLastError());
-    const int grid = grid_for_groups(c, ((int64_t)g->nwords + 63) / 64);
-    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kBlock), 0, s, *g, *st, level, sp,
+    const int grid = grid_for_qgroups(c, ((int64_t)g->nwords + 63) / 64);
+    hipLaunchKernelGGL(k_words_to_queue<true>, dim3(grid), dim3(kQBlock), 0, s, *g, *st, level, sp,
                        (long long)build_max);
     HIPCHK(hipGetLastError());
     return 0;
