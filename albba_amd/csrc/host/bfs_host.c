@@ -959,9 +959,10 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * edges (which bound the next frontier) stay at most queue_max: past it
      * the next level is likely a pull, which needs no queue (a push after it
      * lists the bitmap instead).  The heuristic pushes exactly while the
-     * frontier is at most push_vertex_cap; the learned selector switches to
-     * pulls about an order of magnitude earlier. */
-    const int64_t queue_max = policy == ALBBA_SWITCH_LEARNED ? push_vertex_cap / 8 : push_vertex_cap;
+     * frontier is at most push_vertex_cap; the learned selector turns to
+     * pulls far earlier (on RMAT s24 from about 2 000 frontier vertices). */
+    const int64_t learned_qmax = G->n / 8192 > 1024 ? G->n / 8192 : 1024;
+    const int64_t queue_max = policy == ALBBA_SWITCH_LEARNED ? learned_qmax : push_vertex_cap;
     loop_state ls = {1, 0, 0, 0, 0};
     int small_streak = 0; /* consecutive plain push levels that stayed small */
     int32_t level = 0;
