@@ -106,6 +106,8 @@ typedef struct albd_graph {
     const int32_t *pull_col;    /* [nnz]       */
     const uint64_t *haslen;     /* [nwords] bit set where row_len > 0 */
     const int32_t *pos_of;      /* [n] vertex id -> position */
+    const uint16_t *row_off;    /* [n] row_order[p] - start of p's window, when
+                                 * windows are at most 65 536 rows; else null */
 } albd_graph;
 
 /* Mutable BFS state on the device. */
@@ -221,6 +223,8 @@ ALBD_HIDDEN int albd_gen_csr(albd_ctx *c, const albd_gen_spec *spec, const int32
 /* ---- BFS pieces ---- */
 /* haslen[w] from row_len */
 ALBD_HIDDEN int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen);
+/* row_off from row_order (g->win at most 65 536) */
+ALBD_HIDDEN int albd_build_row_off(albd_ctx *c, const albd_graph *g, uint16_t *row_off);
 /* reset every state array and seed the root (position rp, vertex id ro) */
 ALBD_HIDDEN int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st,
                               int32_t rp, int32_t ro);

@@ -382,6 +382,14 @@ __global__ __launch_bounds__(kBlock) void k_haslen(const int32_t *row_len, uint6
 // which the first select kernel sums.  lp holds stale values outside the
 // visited set (the vertex-order pass masks them), and pmin is all ones at
 // rest (the push finalize restores every entry it consumed).
+// each row's vertex id as an offset inside its window (two bytes where
+// row_order takes four: the vertex-order pass streams these)
+__global__ __launch_bounds__(kBlock) void k_row_off(albd_graph g, uint16_t *row_off)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < g.n) row_off[p] = (uint16_t)(g.row_order[p] - (int32_t)((p / g.win) * g.win));
+}
+
 __global__ __launch_bounds__(kBlock) void k_bfs_reset(albd_graph g, albd_state st, int32_t rp, int32_t ro)
 {
     const int64_t deg = g.push_ptr[rp + 1] - g.push_ptr[rp];
@@ -778,8 +786,8 @@ __global__ __launch_bounds__(kQBlock) void k_words_to_queue(albd_graph g, albd_s
 
 // level and parent from position order to vertex order, once per BFS.
 // Positions [k*win, (k+1)*win) hold the vertices of the same id range (the
-// sigma windows), so a block takes one window: it streams the window's
-// row_order, lp and visited words in position order, scatters level and
+// sigma windows), so a block takes one window: it streams the window's row
+// offsets (row_off), lp and visited words in position order, scatters level and
 // parent into LDS by vertex, and writes both out in vertex order.  Every
 // global access is a coalesced stream.  LDS: 8 bytes per window row.
 constexpr int kMaxWin = 8192;
@@ -801,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
             for (int k = 0; k < kV; ++k) {
                 const int32_t i = i0 + k * kBlock;
                 const int64_t p = lo + (i < len ? i : 0);
-                v[k] = g.row_order[p];
+                v[k] = (int32_t)lo + (int32_t)g.row_off[p];
                 x[k] = st.lp[p];
                 vis[k] = st.visited[p >> 6];
             }
@@ -1326,6 +1334,16 @@ int launch_pull_c(albd_ctx *c, PullArgs a, int variant, size_t lds, int64_t item
 
 extern "C" {
 
+int albd_build_row_off(albd_ctx *c, const albd_graph *g, uint16_t *row_off)
+{
+    if (g->n <= 0) return 0;
+    if (g->win < 1 || g->win > 65536) return albd_set_error_msg(ALBBA_EINVAL, "row offsets: window too long");
+    hipLaunchKernelGGL(k_row_off, dim3((unsigned)(((int64_t)g->n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       albd_stream(c), *g, row_off);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int albd_build_haslen(albd_ctx *c, const albd_graph *g, uint64_t *haslen)
 {
     const int64_t threads = (int64_t)g->nwords * 64;
@@ -1348,7 +1366,7 @@ int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_
 
 int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st)
 {
-    if (g->win >= 1 && g->win <= kMaxWin) {
+    if (g->win >= 1 && g->win <= kMaxWin && g->row_off) {
         int64_t blocks = ((int64_t)g->n + g->win - 1) / g->win;
         int64_t cap = (int64_t)albd_num_cus(c) * 16;
         hipLaunchKernelGGL(k_to_vertex_order_win,

@@ -148,6 +148,7 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, (void *)G->g.push_col);
     albd_free(d, (void *)G->g.haslen);
     albd_free(d, (void *)G->g.pos_of);
+    albd_free(d, (void *)G->g.row_off);
     albd_free(d, G->st.lp);
     albd_free(d, G->st.visited);
     albd_free(d, G->st.front);
@@ -207,8 +208,11 @@ static int graph_state_alloc(albba_graph *G)
     st->persist = (albd_persist *)dalloc(G, sizeof(albd_persist), &rc);
     G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
+    g->row_off = g->win <= 65536 ? (const uint16_t *)dalloc(G, sizeof(uint16_t) * (size_t)(n > 0 ? n : 1), &rc)
+                                 : NULL;
     if (rc) return albba_fail(rc, "%s", albd_error());
     if ((rc = albd_build_haslen(G->ctx->d, g, (uint64_t *)g->haslen)) ||
+        (g->row_off && (rc = albd_build_row_off(G->ctx->d, g, (uint16_t *)g->row_off))) ||
         (rc = albd_fill_u64(G->ctx->d, st->pkey, ~0ull, (size_t)npos)) ||
         (rc = albd_fill_u32(G->ctx->d, st->pmin, 0xFFFFFFFFu, (size_t)npos)) ||
         (rc = albd_sync(G->ctx->d)))
