@@ -125,6 +125,7 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     constexpr int SUB = C < 64 ? C : 64;
     const uint64_t gmask = SUB == 64 ? ~0ull : (((1ull << SUB) - 1) << ((lane / SUB) * SUB));
 
+    const bool sym = g.pull_col == g.push_col;
     unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0, wscan = 0, wbyp = 0;
     // One word ahead: while a word's columns are scanned, the next word is
     // claimed, its list entry read, and then its row lengths, visited word
@@ -313,7 +314,10 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                     if ((found >> lane) & 1ull) {
                         // position order: the lanes of a word write one run
                         a.st.lp[p] = pack_lp(a.level + 1, g.row_order[parent_pos]);
-                        edges += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
+                        // out-degree: the row's length when the matrix is
+                        // symmetric (the pull rows are the push rows)
+                        edges += sym ? (unsigned long long)len
+                                     : (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
                     }
                     cnt += __popcll(found);
                     nzw += 1;
