@@ -217,15 +217,25 @@ struct RowArgs {
     const int64_t *rp;
     const int32_t *col;
     const int32_t *row_order, *row_len, *pos_of;
+    // with power-of-two windows of at most 65 536 rows: a vertex's position
+    // is its window's first id plus a 16-bit offset, and the random gathers
+    // read this half-size table instead of pos_of (null otherwise)
+    const uint16_t *pos_off;
+    int32_t win_mask;
     const int64_t *rm_ptr;  // row-major copy: pointers over positions
     int32_t *rm_col;
     unsigned int *push_cnt; // directed: out-degree histogram over positions, or null
     int32_t n;
 };
 
+__device__ __forceinline__ int32_t position_of(const RowArgs &a, int32_t v)
+{
+    return a.pos_off ? (v & ~a.win_mask) | (int32_t)a.pos_off[v] : a.pos_of[v];
+}
+
 __device__ __forceinline__ void copy_entry(const RowArgs &a, int64_t src, int64_t dst)
 {
-    const int32_t c = a.pos_of[a.col[src]];
+    const int32_t c = position_of(a, a.col[src]);
     a.rm_col[dst] = c;
     if (a.push_cnt) atomicAdd(&a.push_cnt[c], 1u);
 }
@@ -278,7 +288,7 @@ __global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
 #pragma unroll
             for (int u = 0; u < U; ++u) c[u] = act[u] ? __builtin_nontemporal_load(&a.col[src[u]]) : 0;
 #pragma unroll
-            for (int u = 0; u < U; ++u) c[u] = act[u] ? a.pos_of[c[u]] : 0;
+            for (int u = 0; u < U; ++u) c[u] = act[u] ? position_of(a, c[u]) : 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (!act[u]) continue;
@@ -287,6 +297,13 @@ __global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
             }
         }
     }
+}
+
+// the in-window offsets of the vertices' positions (power-of-two windows)
+__global__ __launch_bounds__(kBB) void k_pos_off(const int32_t *pos_of, int32_t n, int32_t win_mask, uint16_t *out)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBB + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBB)
+        out[v] = (uint16_t)(pos_of[v] & win_mask);
 }
 
 // positions of the rows longer than kLongRow
@@ -673,6 +690,16 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     ra.row_order = row_order;
     ra.row_len = row_len;
     ra.pos_of = pos_of;
+    ra.pos_off = nullptr;
+    ra.win_mask = 0;
+    if (sigma >= 1 && sigma <= 65536 && (sigma & (sigma - 1)) == 0 && n > 0) {
+        uint16_t *po = nullptr;
+        if ((rc = T.get(n, &po))) return rc;
+        hipLaunchKernelGGL(k_pos_off, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, pos_of, n, sigma - 1, po);
+        HIPCHK(hipGetLastError());
+        ra.pos_off = po;
+        ra.win_mask = sigma - 1;
+    }
     ra.rm_ptr = rm_ptr;
     ra.rm_col = rm_col;
     ra.push_cnt = cnt;
