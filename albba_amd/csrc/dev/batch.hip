@@ -81,17 +81,17 @@ __device__ void add_counts(BLds &S, long long *slot, unsigned long long f, unsig
     __syncthreads();
 }
 
-// the LDS list of a push window -> the instance's found list (one
-// reservation per window)
-__device__ void flush_found(BLds &S, const albd_batch_item &it)
+// the LDS list of a push window -> the instance's found list at
+// found[base + ...] (one reservation per window on the level's counter)
+__device__ void flush_found(BLds &S, const albd_batch_item &it, unsigned long long *counter, int64_t base)
 {
     __syncthreads();
     const unsigned int nf = S.nf;
-    if (threadIdx.x == 0) S.bf = nf ? atomicAdd(&it.ctl[1], (unsigned long long)nf) : 0;
+    if (threadIdx.x == 0) S.bf = nf ? atomicAdd(counter, (unsigned long long)nf) : 0;
     __syncthreads();
     if (threadIdx.x == 0) S.nf = 0;
     const unsigned long long bf = S.bf;
-    for (unsigned int i = threadIdx.x; i < nf; i += kBB) st_coherent(&it.found[bf + i], S.found[i]);
+    for (unsigned int i = threadIdx.x; i < nf; i += kBB) st_coherent(&it.found[base + bf + i], S.found[i]);
     __syncthreads();
 }
 
@@ -128,7 +128,6 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[rp >> 6]), bit);
         atomicOr(reinterpret_cast<unsigned long long *>(&it.bm[0][rp >> 6]), bit);
         st_coherent(&it.found[0], rp);
-        st_coherent(&it.ctl[1], 1ull);
     }
     if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
 
@@ -192,10 +191,10 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                     if (tid == 0) {
                         unsigned long long tot = 0;
                         for (int i = 0; i < kBWaves; ++i) tot += S.s_cnt[i];
-                        S.s_base = tot ? atomicAdd(&it.ctl[1], tot) : 0ull;
+                        S.s_base = tot ? atomicAdd(reinterpret_cast<unsigned long long *>(&slot[2]), tot) : 0ull;
                     }
                     __syncthreads();
-                    unsigned long long o = S.s_base + ex;
+                    unsigned long long o = (unsigned long long)fe_cur + S.s_base + ex;
                     for (int i = 0; i < wid; ++i) o += S.s_cnt[i];
                     __syncthreads();
                     uint64_t kb = word;
@@ -207,7 +206,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                 }
                 if (!team_sync(bar, err, T, spin_limit, &S.ok)) return false;
                 fs_cur = fe_cur;
-                fe_cur = (int64_t)ld_coherent(&it.ctl[1]);
+                fe_cur += (int64_t)ld_coherent(&slot[2]);
             }
             // expansion: workgroup `rank` takes queue entries [q0, q1)
             const int64_t qlen = fe_cur - fs_cur;
@@ -280,7 +279,7 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
                         const unsigned int pf = wave_append(&S.nf, won[i]);
                         if (won[i]) S.found[pf] = v[i];
                     }
-                    flush_found(S, it);
+                    flush_found(S, it, reinterpret_cast<unsigned long long *>(&slot[0]), fe_cur);
                 }
                 __syncthreads(); // pre/start/ou are restaged
             }
@@ -361,10 +360,12 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
         prev_by_pull = cur_by_pull;
         F_prev = F;
         if (push) {
-            const int64_t total = (int64_t)ld_coherent(&it.ctl[1]);
-            F = (unsigned long long)(total - fe_cur);
+            // the level's appends, counted in its own ring slot: a
+            // workgroup past the barrier may already append for the next
+            // level while another still reads this one's count
+            F = (unsigned long long)ld_coherent(&slot[0]);
             fs_cur = fe_cur;
-            fe_cur = total;
+            fe_cur += (int64_t)F;
         } else {
             F = (unsigned long long)ld_coherent(&slot[0]);
             fs_cur = fe_cur; // a pulled level lists nothing

@@ -1109,7 +1109,11 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
     bool ok = true;
     for (int k = 0; ok; ++k) {
         const uint32_t lvl = (uint32_t)(a.level0 + k + 1);
-        int64_t *cnt = st.level_log + 4 * k; // found, edges, queue entries of level k
+        // found, edges, queue entries of level k, and the level's count in
+        // the found list (appends go to found[fo + ...]).  Per level: a
+        // workgroup past the barrier may already append for the next level
+        // while another still reads this level's counts.
+        int64_t *cnt = st.level_log + 4 * k;
         // expansion: workgroup w takes queue entries [q0, q1)
         const int64_t per = (qlen + G - 1) / G;
         const int64_t q0 = (int64_t)w * per, q1 = q0 + per < qlen ? q0 + per : qlen;
@@ -1198,14 +1202,15 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
                 // flush the window's lists: one reservation per list
                 const unsigned int nf = s_nf, nq = s_nq;
                 if (tid == 0) {
-                    s_bf = nf ? atomicAdd(&P->found_total, (unsigned long long)nf) : 0;
+                    s_bf = nf ? atomicAdd(reinterpret_cast<unsigned long long *>(&cnt[3]), (unsigned long long)nf)
+                              : 0;
                     s_bq = nq ? atomicAdd(reinterpret_cast<unsigned long long *>(&cnt[2]),
                                           (unsigned long long)nq)
                               : 0;
                 }
                 __syncthreads();
                 const unsigned long long bf = s_bf, bq = s_bq;
-                for (unsigned int i = tid; i < nf; i += kSmallBlock) st_coherent(&found[bf + i], s_found[i]);
+                for (unsigned int i = tid; i < nf; i += kSmallBlock) st_coherent(&found[fo + bf + i], s_found[i]);
                 for (unsigned int i = tid; i < nq; i += kSmallBlock) st_coherent(&qb[bq + i], s_queue[i]);
                 __syncthreads();
                 if (tid == 0) {
@@ -1227,8 +1232,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_push_small(SmallArgs a)
         }
         ok = team_sync(&P->barrier, &P->error, G, a.spin_limit, &s_ok);
         if (!ok) break;
-        const int64_t total = (int64_t)ld_coherent(&P->found_total);
-        const int64_t na = total - fo;
+        const int64_t na = ld_coherent(&cnt[3]);
+        const int64_t total = fo + na;
         const int64_t edges = ld_coherent(&cnt[1]);
         const int64_t nq = ld_coherent(&cnt[2]);
         if (w == 0 && tid == 0) st_coherent(&cnt[0], na);
