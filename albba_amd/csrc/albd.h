@@ -71,7 +71,6 @@ typedef struct albd_counters {
 
 /* grid state of the small-frontier kernel, zeroed before each launch */
 typedef struct albd_persist {
-    unsigned long long found_total;  /* vertices found so far by the launch */
     unsigned long long barrier;      /* arrivals at the grid barrier        */
     unsigned int error;              /* a workgroup gave up waiting         */
     unsigned int pad_;

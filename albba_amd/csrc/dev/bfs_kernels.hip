@@ -586,7 +586,6 @@ __global__ __launch_bounds__(kBlock) void k_level_begin(albd_state st, int32_t n
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
             if (flags & LB_SMALL) {
-                st.persist->found_total = 0;
                 st.persist->barrier = 0;
                 st.persist->error = 0;
                 for (int f = 0; f < 4; ++f) st.level_log[f] = 0;
