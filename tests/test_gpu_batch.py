@@ -73,6 +73,16 @@ def test_batch_matches_oracle(batch_setup, mapping, workgroups):
     check_batch(graphs, inst, roots, b, out2)
 
 
+def test_batch_repeated_runs(batch_setup):
+    """Eight runs in a row, every instance checked each time.  A regression
+    test for a race between a team's workgroups: one past a level's barrier
+    appending for the next level while another still read the level's
+    count; it showed as wrong levels on one machine in every run."""
+    graphs, inst, roots, b = batch_setup
+    for rep in range(8):
+        check_batch(graphs, inst, roots, b, b.run(roots, mapping=rep % 2))
+
+
 def test_batch_learned_policy_and_mixed_mappings(ctx, batch_setup):
     graphs, inst, roots, b = batch_setup
     sel = A.Selector(ctx)
