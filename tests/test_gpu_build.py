@@ -69,6 +69,34 @@ def test_medium_graphs_and_bfs(ctx, kind):
         dg.close()
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_position_tables_above_one_l2(ctx, directed):
+    """Graphs of 1.1 M and 2.2 M vertices, whose position tables (16-bit
+    in-window offsets with a power-of-two sigma, 32-bit positions otherwise)
+    outgrow one 4 MB L2: layouts against the host builder, BFS against the
+    oracle."""
+    if directed:
+        base = A.Graph.er(1_100_000, 4.0, seed=133, perm_seed=134)
+        rp, col = base.row_ptr, base.col
+        rows = np.repeat(np.arange(base.n), np.diff(rp))
+        keep = col > rows
+        rp2 = np.zeros(base.n + 1, np.int64)
+        np.cumsum(np.bincount(rows[keep], minlength=base.n), out=rp2[1:])
+        g = A.Graph.from_arrays(rp2, col[keep].astype(np.int32), symmetric=False)
+        shapes = [(64, 960, 1), (32, 4096, 16)]
+    else:
+        g = A.Graph.er(2_200_000, 4.0, seed=131, perm_seed=132)
+        shapes = [(64, 4096, 1), (128, 3072, 32)]
+    for C_, sigma, pad in shapes:
+        dg = assert_same_layout(ctx, g, C_, sigma, pad)
+        root = int(g.pick_roots(1, 135)[0])
+        o = dg.bfs(root)
+        lv, pa = O.bfs_queue(g.row_ptr, g.col, root)
+        np.testing.assert_array_equal(o["level"], lv)
+        np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+
+
 def test_rows_out_of_order_are_sorted(ctx):
     base = A.Graph.rmat(12, 16, seed=101, perm_seed=102)
     rp, col = base.row_ptr.copy(), base.col.copy()
