@@ -804,29 +804,36 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
     for (int64_t w = blockIdx.x; w < nwin; w += gridDim.x) {
         const int64_t lo = w * g.win;
         const int32_t len = (int32_t)((int64_t)g.n - lo < g.win ? (int64_t)g.n - lo : g.win);
+        // unreached vertices keep -1; only reached positions load their
+        // offset and lp (a window's empty rows, its last positions, are
+        // never reached unless one is the root: 43 % of the positions of an
+        // RMAT s22 graph, whose lines are then not read)
+        for (int32_t i = threadIdx.x; i < len; i += kBlock) {
+            s_level[i] = -1;
+            s_parent[i] = -1;
+        }
+        __syncthreads();
         for (int32_t i0 = threadIdx.x; i0 < len; i0 += kBlock * kV) {
-            int32_t v[kV];
-            unsigned long long x[kV];
-            uint64_t vis[kV];
+            bool reached[kV];
 #pragma unroll
             for (int k = 0; k < kV; ++k) {
                 const int32_t i = i0 + k * kBlock;
                 const int64_t p = lo + (i < len ? i : 0);
-                v[k] = (int32_t)lo + (int32_t)g.row_off[p];
-                x[k] = st.lp[p];
-                vis[k] = st.visited[p >> 6];
+                reached[k] = i < len && ((st.visited[p >> 6] >> (p & 63)) & 1ull);
+            }
+            int32_t o[kV];
+            unsigned long long x[kV];
+#pragma unroll
+            for (int k = 0; k < kV; ++k) {
+                const int64_t p = lo + i0 + k * kBlock;
+                o[k] = reached[k] ? (int32_t)g.row_off[p] : 0;
+                x[k] = reached[k] ? st.lp[p] : 0ull;
             }
 #pragma unroll
             for (int k = 0; k < kV; ++k) {
-                const int32_t i = i0 + k * kBlock;
-                if (i < len) {
-                    const int64_t p = lo + i;
-                    const bool reached = (vis[k] >> (p & 63)) & 1ull;
-                    const int32_t o = (int32_t)(v[k] - lo);
-                    if ((uint32_t)o < (uint32_t)len) {
-                        s_level[o] = reached ? (int32_t)(x[k] >> 32) : -1;
-                        s_parent[o] = reached ? (int32_t)(uint32_t)x[k] : -1;
-                    }
+                if (reached[k] && o[k] < len) {
+                    s_level[o[k]] = (int32_t)(x[k] >> 32);
+                    s_parent[o[k]] = (int32_t)(uint32_t)x[k];
                 }
             }
         }
