@@ -26,6 +26,7 @@ CHOICE_PUSH, CHOICE_PULL, CHOICE_PULL_SB, CHOICE_PULL_CB = 0, 1, 2, 3
 CHOICE_NAMES = ("push", "pull", "pull_sb", "pull_cb")
 SWITCH_HEURISTIC, SWITCH_LEARNED, SWITCH_FIXED, SWITCH_SCRIPT, SWITCH_BEST = 0, 1, 2, 3, 4
 GEN_RMAT, GEN_ER, GEN_GRID = 0, 1, 2
+SELL_SORT_FIRST_COL = 1  # ALBBA_SELL_SORT_FIRST_COL: the reference's -s 1 row order
 NUM_FEATURES = 12
 MLP_NPARAM = 12 * 32 + 32 + 32 * 32 + 32 + 32 * 4 + 4
 ERRORS = {
@@ -56,7 +57,8 @@ class _SELL(C.Structure):
                 ("nvops", C.c_int64), ("row_order", I32P), ("pos_of", I32P),
                 ("slice_widths", I32P), ("slice_pointers", I64P), ("vop_pointers", I64P),
                 ("vop_lengths", U8P), ("column_indices", I32P), ("row_lengths", I32P),
-                ("push_ptr", I64P), ("push_col", I32P), ("pull_ptr", I64P), ("pull_col", I32P)]
+                ("push_ptr", I64P), ("push_col", I32P), ("pull_ptr", I64P), ("pull_col", I32P),
+                ("order_flags", C.c_int32)]
 
 
 class SparseMatrixCSR(C.Structure):
@@ -158,6 +160,8 @@ def lib() -> C.CDLL:
         "albba_pick_roots": (C.c_int, [C.POINTER(_CSR), C.c_int, C.c_uint64, C.c_int, I32P]),
         "albba_sellcs_build": (C.c_int, [C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
                                          C.POINTER(C.POINTER(_SELL))]),
+        "albba_sellcs_build_ex": (C.c_int, [C.POINTER(_CSR), C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.POINTER(_SELL))]),
         "albba_sellcs_free": (None, [C.POINTER(_SELL)]),
         "albba_device_count": (C.c_int, [C.POINTER(C.c_int)]),
         "albba_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -167,6 +171,8 @@ def lib() -> C.CDLL:
         "albba_graph_upload": (C.c_int, [C.c_void_p, C.POINTER(_SELL), C.POINTER(C.c_void_p)]),
         "albba_graph_build": (C.c_int, [C.c_void_p, C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
                                         C.POINTER(C.c_void_p), C.POINTER(_BuildReport)]),
+        "albba_graph_build_ex": (C.c_int, [C.c_void_p, C.POINTER(_CSR), C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.POINTER(C.c_void_p), C.POINTER(_BuildReport)]),
         "albba_graph_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.POINTER(_SELL))]),
         "albba_graph_generate": (C.c_int, [C.c_void_p, C.POINTER(_GenParams), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_void_p), C.POINTER(_CSR),
@@ -334,9 +340,13 @@ class Graph:
             _check(k)
         return out[:k].copy()
 
-    def sellcs(self, C_: int = 64, sigma: int = 4096, pad: int = 32) -> "SellCS":
+    def sellcs(self, C_: int = 64, sigma: int = 4096, pad: int = 32,
+               sort_first: bool = False) -> "SellCS":
+        """The device layout built on the host (albba_sellcs_build_ex);
+        sort_first: the reference's -s 1 first-column row order."""
         p = C.POINTER(_SELL)()
-        _check(lib().albba_sellcs_build(C.byref(self._raw), C_, sigma, pad, C.byref(p)))
+        _check(lib().albba_sellcs_build_ex(C.byref(self._raw), C_, sigma, pad,
+                                           SELL_SORT_FIRST_COL if sort_first else 0, C.byref(p)))
         return SellCS(p, self)
 
 
@@ -372,6 +382,7 @@ class SellCS:
             "push_col": _view(r.push_col, r.nnz, np.int32).copy(),
             "pull_ptr": _view(r.pull_ptr, npos + 1, np.int64).copy() if r.pull_ptr else None,
             "pull_col": _view(r.pull_col, r.nnz, np.int32).copy() if r.pull_col else None,
+            "order_flags": int(r.order_flags),
         }
 
 
@@ -435,17 +446,19 @@ class DeviceGraph:
 
     @classmethod
     def build(cls, ctx: Context, graph: "Graph", C_: int = 64, sigma: int = 4096,
-              pad: int = 1) -> "DeviceGraph":
-        """Build the layout on the device from the CSR (albba_graph_build):
-        the same device arrays as DeviceGraph(ctx, graph.sellcs(C_, sigma, pad))."""
+              pad: int = 1, sort_first: bool = False) -> "DeviceGraph":
+        """Build the layout on the device from the CSR (albba_graph_build_ex):
+        the same device arrays as
+        DeviceGraph(ctx, graph.sellcs(C_, sigma, pad, sort_first))."""
         self = cls.__new__(cls)
         self.ctx = ctx
         self.n = graph.n
         self.nnz = int(graph.nnz)
         self._p = C.c_void_p()
         rep = _BuildReport()
-        _check(lib().albba_graph_build(ctx._p, C.byref(graph._raw), C_, sigma, pad, C.byref(self._p),
-                                       C.byref(rep)))
+        _check(lib().albba_graph_build_ex(ctx._p, C.byref(graph._raw), C_, sigma, pad,
+                                          SELL_SORT_FIRST_COL if sort_first else 0, C.byref(self._p),
+                                          C.byref(rep)))
         self.build_report = {"upload_ms": float(rep.upload_ms), "build_ms": float(rep.build_ms),
                              "rows_sorted": bool(rep.rows_sorted)}
         ctx._adopt(self)
@@ -800,8 +813,9 @@ def sellcs_preprocess(n, row_ptr, col, C_=256, sigma=16384, sort_flag=False) -> 
     return out
 
 
-def run_custom_bfs_test(n, row_ptr, col) -> dict:
-    """bfs/src/bfs.h:13 on the GPU (root 0): frontier sizes, iterations, times."""
+def run_custom_bfs_test(n, row_ptr, col, sort_flag: bool = False) -> dict:
+    """bfs/src/bfs.h:13 on the GPU (root 0): frontier sizes, iterations, times;
+    sort_flag: the reference's -s 1 row order for the layout."""
     m, keep = _csr_struct(n, row_ptr, col)
     bfs_x = np.zeros(max(n, 1), np.float64)
     colidx_x = np.zeros(max(n, 1), np.int32)
@@ -813,7 +827,7 @@ def run_custom_bfs_test(n, row_ptr, col) -> dict:
                               colidx_x.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nnzx),
                               frontier.ctypes.data_as(C.POINTER(C.c_int)), C.byref(num_vex),
                               C.byref(conv), iter_times.ctypes.data_as(F64P), C.byref(bfs_t),
-                              C.byref(num_iter), True, False)
+                              C.byref(num_iter), True, bool(sort_flag))
     k = num_iter.value
     return {"num_iter": k, "num_vex": num_vex.value, "frontier": frontier[:k].copy(),
             "iter_times": iter_times[:k].copy(), "bfs_time": bfs_t.value,

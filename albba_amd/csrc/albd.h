@@ -83,6 +83,8 @@ typedef struct albd_select {
     int32_t prev_choice;   /* choice committed for the previous level           */
     int32_t valid_mask;    /* bit c set: choice c may run                        */
     int32_t first;         /* 1 right after init: the root is the frontier       */
+    int32_t forced;        /* >= 0: the policy's choice for this level is this one
+                              (a scripted level); -1: the MLP's, or the heuristic's */
 } albd_select_args;
 
 /* Read-only graph arrays on the device. */
@@ -107,6 +109,10 @@ typedef struct albd_graph {
     const int32_t *pos_of;      /* [n] vertex id -> position */
     const uint16_t *row_off;    /* [n] row_order[p] - start of p's window, when
                                  * windows are at most 65 536 rows; else null */
+    /* win: the sigma window (rows [k*win, (k+1)*win) of the layout hold the
+     * vertices of that id range), or 0 when the rows were ordered across
+     * windows first (the first-column order): then no window holds its own
+     * id range */
 } albd_graph;
 
 /* Mutable BFS state on the device. */
@@ -179,6 +185,8 @@ typedef struct albd_build_input {
     int32_t n, C, sigma, pad;
     int32_t symmetric;      /* 1: the push CSR is the layout's own rows      */
     int32_t on_device;      /* 1: row_ptr/col are device pointers (no upload) */
+    int32_t sort_first;     /* 1: first-column order before the window sort
+                               (ALBBA_SELL_SORT_FIRST_COL)                   */
 } albd_build_input;
 typedef struct albd_build_report {
     float upload_ms;        /* CSR host -> device                            */

@@ -7,12 +7,15 @@
  * then print the frontier sizes and the reference's GTEPS
  * (bfs/src/report_results.c:36) -- using only the reference's entry points.
  *
- *   albba_bfs -m graph.mtx|graph.cbd [-w sigma] [-v 1]
+ *   albba_bfs -m graph.mtx|graph.cbd [-w sigma] [-s 1] [-v 1]
+ *
+ * -s 1 orders the rows by their first column before the sigma-window sort,
+ * as the reference's -s 1 (bfs/src/main.c:44-83, sort_flag).
  *
  * With -r the albba_* API runs instead: BFS from the given roots with a
  * chosen switch policy, results checked as BFS trees on the host.
  *
- *   albba_bfs -m graph.cbd -r 0,17,42 [-l selector.txt] [-C 64] [-s 4096]
+ *   albba_bfs -m graph.cbd -r 0,17,42 [-l selector.txt] [-C 64] [-w 4096] [-s 1]
  */
 #include "albba.h"
 
@@ -21,7 +24,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-static int reference_sequence(char *path)
+static int reference_sequence(char *path, int sort_flag)
 {
     SparseMatrixCSR a;
     memset(&a, 0, sizeof a);
@@ -52,7 +55,7 @@ static int reference_sequence(char *path)
     double conv_time = 0, bfs_time = 0;
     bfs_x[0] = 1;
     run_custom_bfs_test(&w, bfs_x, colidx_x, &nnzx, frontier, &num_vex, &conv_time, iter_times,
-                        &bfs_time, &num_iter, true, false);
+                        &bfs_time, &num_iter, true, sort_flag != 0);
     printf("iteration = %d, bfs runtime = %.3f ms, conversion = %.3f ms\n", num_iter, bfs_time,
            conv_time);
     printf("frontier:");
@@ -64,7 +67,7 @@ static int reference_sequence(char *path)
 }
 
 static int api_sequence(const char *path, const char *roots, const char *selector, int C,
-                        int sigma)
+                        int sigma, int sort_flag)
 {
     albba_csr g;
     int rc = albba_read_graph(path, &g);
@@ -73,7 +76,8 @@ static int api_sequence(const char *path, const char *roots, const char *selecto
     albba_ctx *ctx = NULL;
     albba_graph *G = NULL;
     albba_selector *sel = NULL;
-    if ((rc = albba_sellcs_build(&g, C, sigma, 1, &s)) || (rc = albba_ctx_create(0, &ctx)) ||
+    if ((rc = albba_sellcs_build_ex(&g, C, sigma, 1, sort_flag ? ALBBA_SELL_SORT_FIRST_COL : 0, &s)) ||
+        (rc = albba_ctx_create(0, &ctx)) ||
         (rc = albba_graph_upload(ctx, s, &G))) {
         fprintf(stderr, "%s\n", albba_last_error());
         return 1;
@@ -127,19 +131,19 @@ static int api_sequence(const char *path, const char *roots, const char *selecto
 int main(int argc, char **argv)
 {
     char *path = NULL, *roots = NULL, *selector = NULL;
-    int C = 64, sigma = 4096, opt;
+    int C = 64, sigma = 4096, sort_flag = 0, opt;
     while ((opt = getopt(argc, argv, "m:w:v:r:l:C:s:p:")) != -1) {
         switch (opt) {
         case 'm': path = optarg; break;
-        case 'w': sigma_window = atoi(optarg); break;
+        case 'w': sigma_window = sigma = atoi(optarg); break;
         case 'v': verify = atoi(optarg); break;
         case 'p': perf_analysis = atoi(optarg); break;
         case 'r': roots = optarg; break;
         case 'l': selector = optarg; break;
         case 'C': C = atoi(optarg); break;
-        case 's': sigma = atoi(optarg); break;
+        case 's': sort_flag = atoi(optarg); break;
         default:
-            fprintf(stderr, "usage: %s -m <graph.mtx|.cbd> [-w sigma] [-r r0,r1,...] [-l selector]\n",
+            fprintf(stderr, "usage: %s -m <graph.mtx|.cbd> [-w sigma] [-s 0|1] [-r r0,r1,...] [-l selector]\n",
                     argv[0]);
             return 2;
         }
@@ -148,5 +152,5 @@ int main(int argc, char **argv)
         fprintf(stderr, "You must specify a valid input method (-m)\n");
         return 2;
     }
-    return roots ? api_sequence(path, roots, selector, C, sigma) : reference_sequence(path);
+    return roots ? api_sequence(path, roots, selector, C, sigma, sort_flag) : reference_sequence(path, sort_flag);
 }

@@ -535,7 +535,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
         c.reached_edges = redges;
         const int heur = heuristic_choice(F, g.n);
         c.heur_choice = heur;
-        c.sel_choice = choice >= 0 ? choice : heur;
+        c.sel_choice = sel.forced >= 0 ? sel.forced : (choice >= 0 ? choice : heur);
         for (int i = 0; i < ALBBA_MLP_IN; ++i) c.features[i] = xs[i];
         // hand the level's lists over to the next level: a pushed level's
         // queue becomes the queue (after a pull the queue is rebuilt from

@@ -122,19 +122,44 @@ struct KeySpec {
     int ib, lb; // bits of the id-in-window and length fields
 };
 
-// 3. sort keys and their decoding
-__global__ __launch_bounds__(kBB) void k_keys(const int32_t *len, KeySpec k, unsigned long long *keys)
+// 3a. first-column order (ALBBA_SELL_SORT_FIRST_COL): key (n - 1 - first
+// column) << 32 | id, so that ascending keys are the reference's stable
+// descending sort by first column (bfs/src/sorting_utils.c:255-281); an
+// empty row takes the next stored entry, 0 past the last one
+__global__ __launch_bounds__(kBB) void k_first_keys(const int64_t *rp, const int32_t *col, int32_t n, int64_t nnz,
+                                                    unsigned long long *keys)
 {
-    for (int64_t v = (int64_t)blockIdx.x * kBB + threadIdx.x; v < k.n; v += (int64_t)gridDim.x * kBB) {
-        const unsigned long long w = (unsigned long long)(v / k.sigma);
-        const unsigned long long off = (unsigned long long)(v - (int64_t)w * k.sigma);
-        const unsigned long long inv = (unsigned long long)(k.maxlen - len[v]);
-        keys[v] = (w << (k.lb + k.ib)) | (inv << k.ib) | off;
+    for (int64_t v = (int64_t)blockIdx.x * kBB + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBB) {
+        const int64_t at = rp[v];
+        const int32_t first = at < nnz ? col[at] : 0;
+        keys[v] = ((unsigned long long)(uint32_t)(n - 1 - first) << 32) | (unsigned long long)v;
     }
 }
 
-__global__ __launch_bounds__(kBB) void k_decode(const unsigned long long *keys, KeySpec k, int64_t npos,
-                                                int32_t *row_order, int32_t *row_len, int32_t *pos_of)
+__global__ __launch_bounds__(kBB) void k_low_ids(const unsigned long long *keys, int32_t n, int32_t *pre)
+{
+    for (int64_t r = (int64_t)blockIdx.x * kBB + threadIdx.x; r < n; r += (int64_t)gridDim.x * kBB)
+        pre[r] = (int32_t)(uint32_t)keys[r];
+}
+
+// 3. sort keys and their decoding.  The window sort runs over the rows in
+// the order pre[] (the first-column order), or in id order when pre is null:
+// rank r is row pre[r], and a key carries the rank's offset in its window.
+__global__ __launch_bounds__(kBB) void k_keys(const int32_t *len, const int32_t *pre, KeySpec k,
+                                              unsigned long long *keys)
+{
+    for (int64_t r = (int64_t)blockIdx.x * kBB + threadIdx.x; r < k.n; r += (int64_t)gridDim.x * kBB) {
+        const unsigned long long w = (unsigned long long)(r / k.sigma);
+        const unsigned long long off = (unsigned long long)(r - (int64_t)w * k.sigma);
+        const int32_t v = pre ? pre[r] : (int32_t)r;
+        const unsigned long long inv = (unsigned long long)(k.maxlen - len[v]);
+        keys[r] = (w << (k.lb + k.ib)) | (inv << k.ib) | off;
+    }
+}
+
+__global__ __launch_bounds__(kBB) void k_decode(const unsigned long long *keys, const int32_t *pre, KeySpec k,
+                                                int64_t npos, int32_t *row_order, int32_t *row_len,
+                                                int32_t *pos_of)
 {
     const unsigned long long offmask = k.ib ? (~0ull >> (64 - k.ib)) : 0ull;
     const unsigned long long lenmask = k.lb ? (~0ull >> (64 - k.lb)) : 0ull;
@@ -142,7 +167,8 @@ __global__ __launch_bounds__(kBB) void k_decode(const unsigned long long *keys, 
         if (p < k.n) {
             const unsigned long long key = keys[p];
             const int64_t w = (int64_t)(key >> (k.lb + k.ib));
-            const int32_t v = (int32_t)(w * k.sigma + (int64_t)(key & offmask));
+            const int64_t r = w * k.sigma + (int64_t)(key & offmask);
+            const int32_t v = pre ? pre[r] : (int32_t)r;
             row_order[p] = v;
             row_len[p] = k.maxlen - (int32_t)((key >> k.ib) & lenmask);
             pos_of[v] = (int32_t)p;
@@ -571,7 +597,8 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     g->C = C;
     g->pad = pad;
     g->sparse_lanes = 8;
-    g->win = sigma > 0 ? sigma : 1;
+    // no window holds its own id range after the first-column order
+    g->win = in->sort_first ? 0 : (sigma > 0 ? sigma : 1);
     g->nslices = (int32_t)ns;
     g->nwords = (int32_t)nwords;
     g->npos = npos;
@@ -611,8 +638,22 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     if (end_bit > 64) return albd_set_error_msg(ALBBA_ERANGE, "graph_build: sort key beyond 64 bits");
     unsigned long long *keys = nullptr, *sorted_keys = nullptr;
     if ((rc = T.get(n, &keys)) || (rc = T.get(n, &sorted_keys))) return rc;
+    int32_t *pre = nullptr;
+    if (in->sort_first && n > 1) {
+        if ((rc = T.get(n, &pre))) return rc;
+        hipLaunchKernelGGL(k_first_keys, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, rp, col, n, nnz, keys);
+        HIPCHK(hipGetLastError());
+        size_t bytes = 0;
+        const int fb = 32 + bits_for((uint64_t)(n - 1));
+        ROCCHK(rocprim::radix_sort_keys(nullptr, bytes, keys, sorted_keys, (size_t)n, 0, fb, s));
+        unsigned char *tmp = nullptr;
+        if ((rc = T.get(bytes ? bytes : 1, &tmp))) return rc;
+        ROCCHK(rocprim::radix_sort_keys(tmp, bytes, keys, sorted_keys, (size_t)n, 0, fb, s));
+        hipLaunchKernelGGL(k_low_ids, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, sorted_keys, n, pre);
+        HIPCHK(hipGetLastError());
+    }
     if (maxlen > 0 && n > 1) {
-        hipLaunchKernelGGL(k_keys, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, len, ks, keys);
+        hipLaunchKernelGGL(k_keys, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, len, pre, ks, keys);
         HIPCHK(hipGetLastError());
         size_t bytes = 0;
         ROCCHK(rocprim::radix_sort_keys(nullptr, bytes, keys, sorted_keys, (size_t)n, 0, end_bit, s));
@@ -627,8 +668,8 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         HIPCHK(hipGetLastError());
         ks.sigma = 0x7fffffff;
     }
-    hipLaunchKernelGGL(k_decode, dim3(grid_for(npos, kBB)), dim3(kBB), 0, s, sorted_keys, ks, npos, row_order,
-                       row_len, pos_of);
+    hipLaunchKernelGGL(k_decode, dim3(grid_for(npos, kBB)), dim3(kBB), 0, s, sorted_keys, pre, ks, npos,
+                       row_order, row_len, pos_of);
     HIPCHK(hipGetLastError());
 
     // 4.
@@ -692,7 +733,7 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     ra.pos_of = pos_of;
     ra.pos_off = nullptr;
     ra.win_mask = 0;
-    if (sigma >= 1 && sigma <= 65536 && (sigma & (sigma - 1)) == 0 && n > 0) {
+    if (!in->sort_first && sigma >= 1 && sigma <= 65536 && (sigma & (sigma - 1)) == 0 && n > 0) {
         uint16_t *po = nullptr;
         if ((rc = T.get(n, &po))) return rc;
         hipLaunchKernelGGL(k_pos_off, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, pos_of, n, sigma - 1, po);

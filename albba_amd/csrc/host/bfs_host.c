@@ -208,8 +208,9 @@ static int graph_state_alloc(albba_graph *G)
     st->persist = (albd_persist *)dalloc(G, sizeof(albd_persist), &rc);
     G->scratch_visited = dalloc(G, sizeof(uint64_t) * (size_t)nwords, &rc);
     G->scratch_ctr = dalloc(G, sizeof(albd_counters), &rc);
-    g->row_off = g->win <= 65536 ? (const uint16_t *)dalloc(G, sizeof(uint16_t) * (size_t)(n > 0 ? n : 1), &rc)
-                                 : NULL;
+    g->row_off = g->win >= 1 && g->win <= 65536
+                     ? (const uint16_t *)dalloc(G, sizeof(uint16_t) * (size_t)(n > 0 ? n : 1), &rc)
+                     : NULL;
     if (rc) return albba_fail(rc, "%s", albd_error());
     if ((rc = albd_build_haslen(G->ctx->d, g, (uint64_t *)g->haslen)) ||
         (g->row_off && (rc = albd_build_row_off(G->ctx->d, g, (uint16_t *)g->row_off))) ||
@@ -233,6 +234,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     G->ctx = ctx;
     G->n = s->n;
     G->sigma = s->sigma;
+    G->order_flags = s->order_flags;
     G->nnz = s->nnz;
     const int64_t npos_raw = (int64_t)s->nslices * s->C;
     const int64_t nwords = (npos_raw + 63) / 64;
@@ -243,7 +245,7 @@ int albba_graph_upload(albba_ctx *ctx, const albba_sellcs *s, albba_graph **out)
     g->C = s->C;
     g->pad = s->pad;
     g->sparse_lanes = 8;
-    g->win = s->sigma < s->n ? s->sigma : (s->n > 0 ? s->n : 1);
+    g->win = (s->order_flags & ALBBA_SELL_SORT_FIRST_COL) ? 0 : (s->sigma < s->n ? s->sigma : (s->n > 0 ? s->n : 1));
     g->nslices = s->nslices;
     g->nwords = (int32_t)nwords;
     g->npos = npos;
@@ -313,7 +315,15 @@ static void *build_alloc(void *arg, size_t bytes)
 int albba_graph_build(albba_ctx *ctx, const albba_csr *cg, int C, int sigma, int pad, albba_graph **out,
                       albba_build_report *report)
 {
+    return albba_graph_build_ex(ctx, cg, C, sigma, pad, 0, out, report);
+}
+
+int albba_graph_build_ex(albba_ctx *ctx, const albba_csr *cg, int C, int sigma, int pad, int flags,
+                         albba_graph **out, albba_build_report *report)
+{
     if (!ctx || !cg || !out) return albba_fail(ALBBA_EINVAL, "graph_build: null argument");
+    if (flags & ~ALBBA_SELL_SORT_FIRST_COL)
+        return albba_fail(ALBBA_EINVAL, "graph_build: unknown flags %#x", (unsigned)flags);
     if (C != 16 && C != 32 && C != 64 && C != 128)
         return albba_fail(ALBBA_EINVAL, "graph_build: C must be 16, 32, 64 or 128 (got %d)", C);
     if (pad != 1 && pad != 16 && pad != 32)
@@ -337,10 +347,12 @@ int albba_graph_build(albba_ctx *ctx, const albba_csr *cg, int C, int sigma, int
     G->n = cg->n;
     G->nnz = cg->nnz;
     G->sigma = sigma;
+    G->order_flags = flags;
     albd_build_input in;
     memset(&in, 0, sizeof in);
     in.row_ptr = cg->row_ptr;
     in.col = cg->col;
+    in.sort_first = (flags & ALBBA_SELL_SORT_FIRST_COL) != 0;
     in.nnz = cg->nnz;
     in.n = cg->n;
     in.C = C;
@@ -639,6 +651,7 @@ int albba_graph_layout(albba_ctx *ctx, const albba_graph *G, albba_sellcs **out)
     s->n = g->n;
     s->C = g->C;
     s->sigma = G->sigma;
+    s->order_flags = G->order_flags;
     s->pad = g->pad;
     s->nslices = g->nslices;
     s->nnz = g->nnz;
@@ -934,6 +947,15 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
     return rc;
 }
 
+/* The script's choice for `level`, or -1 past its end or where it names an
+ * unavailable choice (the heuristic's then). */
+static int script_at(const albba_graph *G, const albba_bfs_opts *o, int32_t level)
+{
+    if (o->switch_policy != ALBBA_SWITCH_SCRIPT || !o->script || level < 0 || level >= o->script_len)
+        return -1;
+    return choice_ok(G, o->script[level]) ? o->script[level] : -1;
+}
+
 static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
                    albba_bfs_stats *stats)
 {
@@ -955,7 +977,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     int valid_mask = 0;
     for (int c = 0; c < ALBBA_NUM_CHOICES; ++c) valid_mask |= choice_ok(G, c) << c;
     albd_select_args sel = {policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(opts.selector) : NULL,
-                       0, ALBBA_CHOICE_PUSH, valid_mask, 1};
+                            0, ALBBA_CHOICE_PUSH, valid_mask, 1, script_at(G, &opts, 0)};
 
     /* the heuristic pushes while F <= (int)(n * 0.04) */
     const int64_t push_vertex_cap = (int64_t)(int)((double)G->n * 0.04);
@@ -966,15 +988,25 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * frontier is at most push_vertex_cap; the learned selector turns to
      * pulls far earlier (on RMAT s24 from about 2 000 frontier vertices). */
     const int64_t learned_qmax = G->n / 8192 > 1024 ? G->n / 8192 : 1024;
-    const int64_t queue_max = policy == ALBBA_SWITCH_LEARNED ? learned_qmax : push_vertex_cap;
+    const int64_t queue_max_policy = policy == ALBBA_SWITCH_LEARNED ? learned_qmax : push_vertex_cap;
+    /* a scripted level lists the queue exactly when the script pushes next
+     * (-1: never; past the script the heuristic's rule) */
+#define QUEUE_MAX(L)                                                                                   \
+    (policy != ALBBA_SWITCH_SCRIPT || script_at(G, &opts, (L) + 1) < 0                                 \
+         ? queue_max_policy                                                                            \
+         : (script_at(G, &opts, (L) + 1) == ALBBA_CHOICE_PUSH ? INT64_MAX : (int64_t)-1))
     loop_state ls = {1, 0, 0, 0, 0};
     int small_streak = 0; /* consecutive plain push levels that stayed small */
     int32_t level = 0;
     int rc;
-    /* The heuristic and learned policies prepare every level speculatively
-     * beside the select kernel (select_prep); the others prepare each level
-     * after their choice (run_level). */
-    const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED;
+    /* The heuristic, learned and scripted policies prepare every level
+     * speculatively beside the select kernel (select_prep), which also
+     * writes their choice (sel_choice: the MLP's or the script's, else the
+     * heuristic's); the others prepare each level after their choice
+     * (run_level). */
+    const int spec = policy == ALBBA_SWITCH_HEURISTIC || policy == ALBBA_SWITCH_LEARNED ||
+                     policy == ALBBA_SWITCH_SCRIPT;
+    const int use_sel = policy != ALBBA_SWITCH_HEURISTIC; /* kernels queued ahead read sel_choice */
     unsigned int seq = 0;
     /* kernels already queued for the coming level: the choice whose pull is
      * queued, or -1; whether the push kernels are */
@@ -986,9 +1018,8 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         rc = select_prep(G, &ls, &sel, &seq) ||
              /* the root level is a push as a rule; the small-frontier
               * kernel needs two push levels before it */
-             (!opts.time_levels && (queued_push = 1, albd_push_spec(d, &G->g, &G->st, 0,
-                                                                    policy == ALBBA_SWITCH_LEARNED,
-                                                                    queue_max))) ||
+             (!opts.time_levels && (queued_push = 1, albd_push_spec(d, &G->g, &G->st, 0, use_sel,
+                                                                    QUEUE_MAX(0)))) ||
              albd_mail_wait(d, &G->st, seq);
     else
         rc = albd_select_read(d, &G->g, &G->st, &sel);
@@ -999,15 +1030,14 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     while (frontier > 0) {
         float feats[ALBBA_NUM_FEATURES];
         memcpy(feats, hc->features, sizeof feats);
-        /* heuristic and learned choices come from the select kernel (the
-         * learned one already falls back to the heuristic) */
-        int choice = policy == ALBBA_SWITCH_LEARNED ? hc->sel_choice : hc->heur_choice;
+        /* heuristic, learned and scripted choices come from the select
+         * kernel (the learned and scripted ones already fall back to the
+         * heuristic) */
+        int choice = use_sel ? hc->sel_choice : hc->heur_choice;
         float choice_ms[ALBBA_NUM_CHOICES] = {0, 0, 0, 0};
+        const int64_t queue_max = QUEUE_MAX(level);
         if (policy == ALBBA_SWITCH_FIXED) {
             choice = opts.fixed_choice;
-        } else if (policy == ALBBA_SWITCH_SCRIPT && opts.script && level < opts.script_len &&
-                   choice_ok(G, opts.script[level])) {
-            choice = opts.script[level];
         } else if (policy == ALBBA_SWITCH_BEST) {
             /* time every choice on a snapshot of the state, commit the fastest */
             if ((rc = albd_d2d(d, G->scratch_visited, G->st.visited, sizeof(uint64_t) * (size_t)G->g.nwords)) ||
@@ -1082,6 +1112,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         sel.level = level + nlev;
         sel.prev_choice = choice;
         sel.first = 0;
+        sel.forced = script_at(G, &opts, level + nlev);
         if (spec) {
             /* swap to the next level's state, post the select kernel with
              * the next level's preparation beside it, then wait for the
@@ -1107,9 +1138,16 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                 else if (small_streak >= 2)
                     guess = -1;
             }
+            if (sel.forced >= 0) {
+                /* a scripted level is known: queue it, except a push that
+                 * would need the queue listed from the bitmap first */
+                guess = sel.forced;
+                if (guess == ALBBA_CHOICE_PUSH && !(choice == ALBBA_CHOICE_PUSH && !small && queue_built))
+                    guess = -1;
+            }
             if (guess == ALBBA_CHOICE_PUSH && !opts.time_levels) {
-                if ((rc = albd_push_spec(d, &G->g, &G->st, level + nlev, policy == ALBBA_SWITCH_LEARNED,
-                                         queue_max)))
+                if ((rc = albd_push_spec(d, &G->g, &G->st, level + nlev, use_sel,
+                                         QUEUE_MAX(level + nlev))))
                     return albba_fail(rc, "bfs: %s", albd_error());
                 queued_push = 1;
             }
@@ -1121,7 +1159,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                  * the list the preparation builds */
                 if ((rc = albd_pull_spec(d, &G->g, &G->st, variant, level + nlev,
                                          ls.list_valid && ls.list_len > 0 ? ls.list_len : G->g.nwords,
-                                         policy == ALBBA_SWITCH_LEARNED)))
+                                         use_sel)))
                     return albba_fail(rc, "bfs: %s", albd_error());
                 queued_pull = guess;
             }
@@ -1180,6 +1218,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     }
     return ALBBA_OK;
 }
+#undef QUEUE_MAX
 
 int albba_bfs_results(albba_ctx *ctx, albba_graph *G, int32_t *level, int32_t *parent)
 {
@@ -1200,8 +1239,7 @@ void run_custom_bfs_test(const SparseMatrixCSR *matrix, elem_t *bfs_x, int *coli
 {
     (void)bfs_x;
     (void)colidx_x;
-    (void)bit_flag;
-    (void)sort_flag;
+    (void)bit_flag; /* the frontier is always a bitmap here */
     const int n = matrix->nrows;
     albba_csr g;
     g.n = n;
@@ -1214,7 +1252,8 @@ void run_custom_bfs_test(const SparseMatrixCSR *matrix, elem_t *bfs_x, int *coli
     int sigma = sigma_window < 64 ? 64 : (sigma_window / 64) * 64;
     double t0 = albba_now_ms();
     albba_sellcs *s = NULL;
-    int rc = albba_sellcs_build(&g, 64, sigma, 32, &s);
+    /* sort_flag: the reference's -s 1 row order (bfs/src/bfs_SELLCSv2.c:40-45) */
+    int rc = albba_sellcs_build_ex(&g, 64, sigma, 1, sort_flag ? ALBBA_SELL_SORT_FIRST_COL : 0, &s);
     *conv_time = albba_now_ms() - t0;
     free(g.row_ptr);
     albba_ctx *ctx = NULL;

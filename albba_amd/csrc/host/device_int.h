@@ -17,6 +17,7 @@ struct albba_graph {
     albd_state st;
     int32_t n;
     int32_t sigma;       /* the layout's sigma as requested */
+    int32_t order_flags; /* ALBBA_SELL_* row-order flags of the layout */
     int64_t nnz;
     int32_t *pos_of;     /* host: vertex id -> position */
     int64_t device_bytes;

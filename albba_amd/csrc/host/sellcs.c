@@ -271,9 +271,38 @@ static int cmp_i32(const void *x, const void *y)
 
 static inline int64_t pad_up(int64_t x, int64_t p) { return (x + p - 1) / p * p; }
 
+/* The row order before the sigma-window sort under ALBBA_SELL_SORT_FIRST_COL
+ * (bfs/src/sorting_utils.c:255-281): rows by first column index, descending,
+ * ties in ascending id.  col holds rows in ascending order, so a row's first
+ * entry is its smallest in-neighbour; an empty row reads the next stored
+ * entry as the reference does, and 0 past the last one. */
+static int first_col_order(const albba_csr *g, const int32_t *col, int32_t *pre)
+{
+    const int32_t n = g->n;
+    int32_t *key = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
+    int32_t *tk = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
+    int32_t *ti = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
+    if (!key || !tk || !ti) { free(key); free(tk); free(ti); return ALBBA_ENOMEM; }
+    for (int32_t v = 0; v < n; ++v) {
+        const int64_t at = g->row_ptr[v];
+        key[v] = at < g->nnz ? col[at] : 0;
+        pre[v] = v;
+    }
+    albba_sort_window_desc(key, pre, 0, n, tk, ti);
+    free(key); free(tk); free(ti);
+    return ALBBA_OK;
+}
+
 int albba_sellcs_build(const albba_csr *g, int C, int sigma, int pad, albba_sellcs **out)
 {
+    return albba_sellcs_build_ex(g, C, sigma, pad, 0, out);
+}
+
+int albba_sellcs_build_ex(const albba_csr *g, int C, int sigma, int pad, int flags, albba_sellcs **out)
+{
     if (!g || !out) return albba_fail(ALBBA_EINVAL, "sellcs_build: null argument");
+    if (flags & ~ALBBA_SELL_SORT_FIRST_COL)
+        return albba_fail(ALBBA_EINVAL, "sellcs_build: unknown flags %#x", (unsigned)flags);
     if (C != 16 && C != 32 && C != 64 && C != 128)
         return albba_fail(ALBBA_EINVAL, "sellcs_build: C must be 16, 32, 64 or 128 (got %d)", C);
     if (pad != 1 && pad != 16 && pad != 32)
@@ -303,6 +332,7 @@ int albba_sellcs_build(const albba_csr *g, int C, int sigma, int pad, albba_sell
     albba_sellcs *s = (albba_sellcs *)albba_calloc(1, sizeof(albba_sellcs));
     if (!s) { free(sorted_copy); return albba_fail(ALBBA_ENOMEM, "sellcs_build"); }
     s->n = n; s->C = C; s->sigma = sigma; s->pad = pad; s->nslices = ns; s->nnz = g->nnz;
+    s->order_flags = flags;
     int32_t *len = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
     s->row_order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
     s->pos_of = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)n);
@@ -315,8 +345,23 @@ int albba_sellcs_build(const albba_csr *g, int C, int sigma, int pad, albba_sell
         free(len); free(sorted_copy); albba_sellcs_free(s);
         return albba_fail(ALBBA_ENOMEM, "sellcs_build: metadata");
     }
-    for (int32_t i = 0; i < n; ++i) len[i] = (int32_t)(g->row_ptr[i + 1] - g->row_ptr[i]);
-    int rc = albba_sigma_order(len, n, sigma, s->row_order, s->row_lengths);
+    int rc = ALBBA_OK;
+    if (flags & ALBBA_SELL_SORT_FIRST_COL) {
+        /* window sort over the first-column order: position p of the
+         * pre-order holds row pre[p] (pos_of is free until the end) */
+        int32_t *pre = s->pos_of;
+        if ((rc = first_col_order(g, gcol, pre))) {
+            free(len); free(sorted_copy); albba_sellcs_free(s);
+            return albba_fail(rc, "sellcs_build: first-column order");
+        }
+        for (int32_t i = 0; i < n; ++i) len[i] = (int32_t)(g->row_ptr[pre[i] + 1] - g->row_ptr[pre[i]]);
+        rc = albba_sigma_order(len, n, sigma, s->row_order, s->row_lengths);
+        for (int32_t p = 0; !rc && p < n; ++p) len[p] = pre[s->row_order[p]];
+        if (!rc) memcpy(s->row_order, len, sizeof(int32_t) * (size_t)n);
+    } else {
+        for (int32_t i = 0; i < n; ++i) len[i] = (int32_t)(g->row_ptr[i + 1] - g->row_ptr[i]);
+        rc = albba_sigma_order(len, n, sigma, s->row_order, s->row_lengths);
+    }
     free(len);
     if (rc) { free(sorted_copy); albba_sellcs_free(s); return rc; }
     for (int32_t p = 0; p < n; ++p) s->pos_of[s->row_order[p]] = p;

@@ -46,10 +46,11 @@ KERNEL_OF_CHOICE = {0: "k_push_expand", 1: "k_pull<64, 0>", 2: "k_pull<64, 1>",
                     3: "k_pull<64, 2>"}
 
 
-def run_policy(dg, roots, policy, sel, steps):
+def run_policy(dg, roots, policy, sel, steps, scripts=None):
     """Time `steps` passes over all roots (no per-level events); returns the
     per-root mean BFS ms (events around each BFS), the mean step ms and the
-    stored entries of each root's component."""
+    stored entries of each root's component.  scripts[i]: the per-level
+    choices of root i (the scripted policy)."""
     per_root = np.zeros(len(roots))
     comp = np.zeros(len(roots))
     step_ms = []
@@ -57,7 +58,8 @@ def run_policy(dg, roots, policy, sel, steps):
         dg.ctx.synchronize()
         t0 = time.perf_counter()
         for i, r in enumerate(roots):
-            o = dg.bfs(int(r), policy=policy, selector=sel, results=False)
+            o = dg.bfs(int(r), policy=policy, selector=sel, results=False,
+                       script=None if scripts is None else scripts[i])
             per_root[i] += o["bfs_ms"] / steps
             comp[i] = o["edges_in_component"]
         dg.ctx.synchronize()
@@ -146,323 +148,84 @@ def compare_builders(A, ctx, g):
     return out
 
 
-def cpu_baseline(g, roots, gpu_levels):
-    """The oracle's C + OpenMP restatement of ALBBA (reference C=256,
-    sigma=16384, tasks = threads) on a bounded sample of the same roots."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_alba_sample(g, roots, budget_s=12.0, max_roots=4, reps=3, gpu_levels=None):
+    """The oracle's C + OpenMP restatement of ALBBA (reference layout C=256,
+    sigma=16384) on a bounded sample of roots: the reference's fixed split
+    into 8 tasks (bfs/src/bfs_SELLCSv2.c:213-214) and tasks = threads, the
+    median of `reps` runs per root, harmonic-mean GTEPS.  gpu_levels[i] =
+    (level, parent) of root i on the GPU, compared when given."""
     from oracle import oracle as O
+    import albba_amd as A
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    rp, col = g.row_ptr, g.col
+    threads = host_threads()
     t = time.time()
-    s = O.build_sell(rp, col, 256, 16384)
+    s = O.build_sell(g.row_ptr, g.col, 256, 16384)
     log(f"cpu baseline: oracle SELL build {time.time() - t:.1f}s")
-    times, parity = [], True
-    budget_s, spent = 20.0, 0.0
-    for i, r in enumerate(roots):
-        t0 = time.perf_counter()
-        res = O.albba_bfs_on(s, g.n, int(r), tasks=threads)
-        dt = time.perf_counter() - t0
-        times.append(dt * 1e3)
-        spent += dt
-        if i in gpu_levels:
-            parity &= bool(np.array_equal(res["level"], gpu_levels[i][0]) and
-                           np.array_equal(res["parent"], gpu_levels[i][1]))
-        log(f"cpu baseline: root {r} {dt * 1e3:.0f} ms")
-        if spent > budget_s or i >= 7:
-            break
-    O.free_sell(s)
-    import albba_amd as A
-
-    gt = [A.gteps(g.nnz, ms) for ms in times]
-    return {"value": round(A.harmonic_mean(gt), 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
-            "sample": f"ALBBA level loop restated in C+OpenMP (oracle), {len(times)} of the same "
-                      f"roots, same RMAT s24 graph, SELL C=256 sigma=16384, {threads} tasks; "
-                      f"harmonic mean", "levels_match_gpu": parity}
-
-
-def batch_specs(count, mix_seed):
-    """Seeded instance mix: RMAT s14-20, ef 8-32, a 0.45-0.65 (b = c =
-    (1 - a - 0.05) / 2); Erdos-Renyi n = 2^16..2^20, mean degree 4-16; square
-    4-neighbour grids of side 256..2048 with 10% of the edges removed."""
-    import albba_amd as A
-
-    def u01(x):
-        return (x >> 11) / float(1 << 53)
-
-    specs = []
-    for k in range(count):
-        r = [A.rng_u64(mix_seed, 16 * k + i) for i in range(8)]
-        kind = r[0] % 3
-        if kind == 0:
-            specs.append(("rmat", dict(scale=14 + r[1] % 7, edgefactor=8 + r[2] % 25,
-                                       a=round(0.45 + 0.2 * u01(r[3]), 4),
-                                       seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
-        elif kind == 1:
-            specs.append(("er", dict(n=1 << (16 + r[1] % 5), mean_degree=round(4 + 12 * u01(r[2]), 3),
-                                     seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
-        else:
-            side = 256 + r[1] % 1793
-            specs.append(("grid", dict(rows=side, cols=side, p_remove=0.1,
-                                       seed=r[4] % 1000003 + 1, perm_seed=r[5] % 1000003 + 1)))
-    return specs
+    variants = {"tasks_8": 8, "tasks_threads": threads}
+    ms = {k: [] for k in variants}
+    parity, spent = True, 0.0
+    try:
+        for i, r in enumerate(roots[:max_roots]):
+            for name, tasks in variants.items():
+                runs = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    res = O.albba_bfs_on(s, g.n, int(r), tasks=tasks)
+                    runs.append((time.perf_counter() - t0) * 1e3)
+                    if gpu_levels is not None and i in gpu_levels:
+                        parity &= bool(np.array_equal(res["level"], gpu_levels[i][0]) and
+                                       np.array_equal(res["parent"], gpu_levels[i][1]))
+                ms[name].append(float(np.median(runs)))
+                spent += sum(runs) / 1e3
+            log(f"cpu baseline: root {r}: 8 tasks {ms['tasks_8'][-1]:.0f} ms, "
+                f"{threads} tasks {ms['tasks_threads'][-1]:.0f} ms (median of {reps})")
+            if spent > budget_s:
+                break
+    finally:
+        O.free_sell(s)
+    out = {k: {"hm_gteps": round(A.harmonic_mean([A.gteps(g.nnz, x) for x in v]), 4),
+               "median_ms_per_root": [round(x, 2) for x in v]} for k, v in ms.items()}
+    return out, len(ms["tasks_8"]), parity
 
 
-def make_graph(kind, p):
-    import albba_amd as A
-
-    if kind == "rmat":
-        b = (1.0 - p["a"] - 0.05) / 2.0
-        return A.Graph.rmat(p["scale"], p["edgefactor"], p["a"], b, b, seed=p["seed"],
-                            perm_seed=p["perm_seed"])
-    if kind == "er":
-        return A.Graph.er(p["n"], p["mean_degree"], seed=p["seed"], perm_seed=p["perm_seed"])
-    return A.Graph.grid(p["rows"], p["cols"], p["p_remove"], seed=p["seed"], perm_seed=p["perm_seed"])
+def cpu_baseline(g, roots, gpu_levels):
+    """CPU baseline of the headline workload (bounded sample of its roots)."""
+    threads = host_threads()
+    var, k, parity = cpu_alba_sample(g, roots, gpu_levels=gpu_levels)
+    return {"value": var["tasks_threads"]["hm_gteps"], "unit": "GTEPS", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"ALBBA level loop restated in C+OpenMP (oracle), the first {k} of the same "
+                      f"roots on the same RMAT s24 graph, SELL C=256 sigma=16384, median of 3 runs "
+                      f"per root, harmonic mean; value = {threads} tasks (one per thread)",
+            "variants": var, "levels_match_gpu": parity}
 
 
 def run_batch_workload(args):
-    """Batched multi-instance BFS: every instance in one cooperative launch.
-    Sampled instances are checked bit for bit against the single-BFS path."""
+    """Batched multi-instance BFS: every instance in one launch.  Layouts from
+    the autotuner per (family, size class); the heuristic and the learned
+    direction policies on the same instances; sampled instances checked bit
+    for bit against the single-BFS path (every instance is checked against
+    the oracle by the GPU test of this workload)."""
     import albba_amd as A
+    from albba_amd import workloads as W
 
-    count, mix_seed = args.instances, 6
-    specs = batch_specs(count, mix_seed)
+    count = args.instances
+    specs = W.batch_specs(count)
     ctx = A.Context(0)
-    sel = None
-    if os.path.exists(SELECTOR):
-        sel = A.Selector(ctx)
-        sel.load(SELECTOR)
-    # the selector is trained on RMAT levels; the batch defaults to the heuristic
-    policy = A.SWITCH_LEARNED if sel is not None and args.policy == "learned" else A.SWITCH_HEURISTIC
-    if policy == A.SWITCH_HEURISTIC:
-        sel = None
-    # layout per graph family from the autotuner, on the family's smallest instance
-    t = time.time()
-    tuned = {}
-    for kind in ("rmat", "er", "grid"):
-        idx = [i for i, (k, _) in enumerate(specs) if k == kind]
-        if not idx:
-            continue
-        small = min(idx, key=lambda i: (specs[i][1].get("scale", 0), specs[i][1].get("n", 0),
-                                        specs[i][1].get("rows", 0)))
-        g = make_graph(*specs[small])
-        roots = g.pick_roots(4, 5, largest_cc=(kind == "grid"))
-        tuned[kind] = A.autotune(ctx, g, roots, policy=policy, selector=sel)["best"]
-        log(f"autotune {kind}: C={tuned[kind]['C']} sigma={tuned[kind]['sigma']} "
-            f"mapping={tuned[kind]['mapping']} ({tuned[kind]['ms']:.3f} ms/BFS)")
-    log(f"autotuning {time.time() - t:.1f}s")
-    t = time.time()
-    dgs, roots, nnz, kinds, keep = [], [], [], [], {}
-    for i, (kind, p) in enumerate(specs):
-        g = make_graph(kind, p)
-        cfg = tuned[kind]
-        sigma = cfg["sigma"] if cfg["sigma"] in (cfg["C"], 256, 4096) else max(g.n, cfg["C"])
-        sell = g.sellcs(cfg["C"], sigma, 1)
-        dgs.append(A.DeviceGraph(ctx, sell))
-        del sell
-        r = g.pick_roots(1, 1000 + i, largest_cc=(kind == "grid"))
-        roots.append(int(r[0]))
-        nnz.append(g.nnz)
-        kinds.append(kind)
-        if i < 3 or i % 64 == 0:
-            keep[i] = dgs[-1]  # checked against the single-BFS path below
-        if (i + 1) % 32 == 0:
-            log(f"  {i + 1}/{count} instances built ({time.time() - t:.1f}s)")
-    mapping = [tuned[k]["mapping"] for k in kinds]
-    b = A.Batch(ctx, dgs)
-    out = b.run(roots, policy=policy, selector=sel, mapping=mapping)  # warm-up
-    exact = True
-    for i, dg in keep.items():
-        lv, pa = b.results(i)
-        o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
-        exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
-    del keep
-    log(f"warm-up done, sampled instances bit-exact: {exact}")
-    runs = []
-    for s in range(args.steps):
-        out = b.run(roots, policy=policy, selector=sel, mapping=mapping)
-        runs.append(out)
-        log(f"  step {s}: {out['total_ms']:.2f} ms")
-    ms = float(np.mean([o["total_ms"] for o in runs]))
-    total = float(np.sum(nnz))
-    done = np.concatenate([o["done_ms"] for o in runs])
-    serv = np.concatenate([o["ms"] for o in runs])
-    line = {
-        "metric": f"batched BFS total GTEPS, {count} instances (RMAT s14-20, ER 2^16-2^20, "
-                  "grids 256^2-2048^2), one cooperative launch",
-        "value": round(total / (ms * 1e6), 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic: seeded mix (splitmix64 counter RNG), one root per instance",
-        "config": {"workload": f"batch{count}_mix{mix_seed}", "instances": count,
-                   "rmat": kinds.count("rmat"), "er": kinds.count("er"), "grid": kinds.count("grid"),
-                   "nnz_stored_total": int(total), "policy": "learned" if sel else "heuristic",
-                   "layouts": tuned},
-        "latency_ms": {"p50": round(float(np.percentile(done, 50)), 3),
-                       "p99": round(float(np.percentile(done, 99)), 3),
-                       "service_p50": round(float(np.percentile(serv, 50)), 3),
-                       "service_p99": round(float(np.percentile(serv, 99)), 3)},
-        "levels_max": int(max(o["num_levels"].max() for o in runs)),
-        "sampled_bit_exact": exact,
-    }
-    print(json.dumps(line), flush=True)
-    b.close()
-    for dg in dgs:
-        dg.close()
-    ctx.close()
-
-
-def run_grid_workload(args):
-    """High-diameter case: 4-neighbour grid of 4096 x 4096 vertices with 10%
-    of the edges removed (seed 4, ids permuted with seed 2), 32 roots from
-    the largest component (seed 5)."""
-    import albba_amd as A
-
-    t = time.time()
-    g = A.Graph.grid(4096, 4096, 0.1, seed=4, perm_seed=2)
-    roots = g.pick_roots(args.roots if args.roots != 64 else 32, 5, largest_cc=True)
-    ctx = A.Context(0)
-    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
-    log(f"grid 4096^2: n={g.n} nnz={g.nnz} ({time.time() - t:.1f}s)")
-    res = {}
-    for name, cap in (("small_frontier_kernel", None), ("per_level_launches", 0)):
-        dg.bfs(int(roots[0]), small_frontier_edges=cap, results=False)  # warm-up
-        ms, levels = [], []
-        for r in roots:
-            o = dg.bfs(int(r), small_frontier_edges=cap, results=False, capacity=1 << 15)
-            ms.append(o["bfs_ms"])
-            levels.append(o["num_levels"])
-        res[name] = {"hm_gteps": A.harmonic_mean([A.gteps(g.nnz, x) for x in ms]),
-                     "mean_ms": float(np.mean(ms)), "levels_mean": float(np.mean(levels)),
-                     "us_per_level": float(np.sum(ms) * 1e3 / np.sum(levels))}
-        log(f"  {name}: {res[name]}")
-        if args.steps <= 1:
-            break
-    R = res["small_frontier_kernel"]
-    line = {
-        "metric": "harmonic-mean GTEPS, 32 roots, 4096^2 grid (10% edges removed), heuristic switch",
-        "value": round(R["hm_gteps"], 4), "unit": "GTEPS", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(R["mean_ms"] * len(roots), 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic: seeded grid, permuted ids",
-        "config": {"workload": "grid4096_p0.1_32roots", "n": g.n, "nnz_stored": g.nnz,
-                   "roots": len(roots), "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
-        "detail": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()},
-    }
-    print(json.dumps(line), flush=True)
-    dg.close()
-    ctx.close()
-
-
-def run_load_workload(args):
-    """.cbd file -> device graph ready for BFS (SELL C=64, sigma=4096,
-    pad=1): the host path (read_Dmatrix + the W = A^T transpose + the host
-    SELL build + upload) against albba_graph_load (file streamed into device
-    memory, transpose and build on the device).  The file is written first,
-    so both read it from the page cache.  RMAT of --scale (default 22 here)."""
-    import shutil
-    import tempfile
-
-    import albba_amd as A
-
-    scale = args.scale if args.scale != 24 else 22
-    g = A.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, perm_seed=2)
-    d = tempfile.mkdtemp(prefix="albba_load_")
-    path = os.path.join(d, f"rmat{scale}.cbd")
-    try:
-        g.write_cbd(path)
-        size = os.path.getsize(path)
-        log(f"wrote {path}: {size / 2**30:.2f} GiB")
-        ctx = A.Context(0)
-        runs = {"host": [], "device": []}
-        same = None
-        for rep_i in range(max(args.steps, 1) + 1):  # the first pair is a warm-up
-            t = time.perf_counter()
-            h = A.Graph.read(path)
-            t_read = time.perf_counter()
-            sell = h.sellcs(64, 4096, 1)
-            t_build = time.perf_counter()
-            dh = A.DeviceGraph(ctx, sell)
-            t_up = time.perf_counter()
-            host = {"read_ms": (t_read - t) * 1e3, "build_ms": (t_build - t_read) * 1e3,
-                    "upload_ms": (t_up - t_build) * 1e3, "total_ms": (t_up - t) * 1e3}
-            t = time.perf_counter()
-            dd = A.DeviceGraph.load(ctx, path, 64, 4096, 1)
-            dev = dict(dd.build_report, wall_ms=(time.perf_counter() - t) * 1e3)
-            if same is None:
-                ref, got = sell.arrays(), dd.layout().arrays()
-                same = all((v is None and got[k] is None) or
-                           (v is not None and np.array_equal(v, got[k])) for k, v in ref.items())
-                del ref, got
-            del sell, h
-            dh.close()
-            dd.close()
-            log(f"  host {host['total_ms']:.0f} ms, device {dev['wall_ms']:.0f} ms ({dev})")
-            if rep_i > 0:
-                runs["host"].append(host)
-                runs["device"].append(dev)
-        mean = {k: {kk: round(float(np.mean([r[kk] for r in v])), 2) for kk in v[0]}
-                for k, v in runs.items()}
-        line = {
-            "metric": f"ms from a .cbd file to a device graph ready for BFS, RMAT s{scale}",
-            "value": mean["device"]["wall_ms"], "unit": "ms", "n_gpus": 1, "steps": len(runs["host"]),
-            "warmup": 1, "ms_per_step": mean["device"]["wall_ms"], "higher_is_better": False,
-            "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-            "data": "synthetic: seeded RMAT written as .cbd (page cache)",
-            "config": {"workload": f"cbd_load_rmat_s{scale}", "n": g.n, "nnz_stored": g.nnz,
-                       "file_bytes": size, "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
-            "host_path_ms": mean["host"]["total_ms"], "detail": mean, "identical": bool(same),
-        }
-        print(json.dumps(line), flush=True)
-        ctx.close()
-    finally:
-        shutil.rmtree(d, ignore_errors=True)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
-    ap.add_argument("--roots", type=int, default=64)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["rmat24", "grid", "batch", "load"], default="rmat24",
-                    help="rmat24: the headline metric; grid: the 4096^2 high-diameter case; "
-                         "batch: the batched multi-instance BFS; load: .cbd file to device graph")
-    ap.add_argument("--instances", type=int, default=256, help="batch workload size")
-    ap.add_argument("--policy", choices=["both", "heuristic", "learned"], default="both",
-                    help="run one policy only (profiling); 'both' checks bit-exactness")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a single BFS would shard as a 1D partition of the SELL slices)",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "batch":
-        run_batch_workload(args)
-        return
-    if args.workload == "grid":
-        run_grid_workload(args)
-        return
-    if args.workload == "load":
-        run_load_workload(args)
-        return
-
-    import albba_amd as A
-
-    ctx = A.Context(0)
-    # the graph generated on the device (the host generator's CSR, bit for
-    # bit), with a host copy for the roots, the builders' comparison and the
-    # CPU baseline
-    dg, g = A.DeviceGraph.generate(ctx, "rmat", 64, 4096, 1, scale=args.scale, edgefactor=16,
-                                   a=0.57, b=0.19, c=0.19, seed=1, perm_seed=2)
-    gen_report = dg.build_report
-    log(f"graph RMAT s{args.scale}: n={g.n} nnz={g.nnz}, generated and laid out on the device "
-        f"({gen_report}, {dg.device_bytes / 2**30:.2f} GiB on device)")
-    roots = g.pick_roots(args.roots, 7)
-    sell_build = compare_builders(A, ctx, g)
-
     sel = None
     if os.path.exists(SELECTOR):
         sel = A.Selector(ctx)
@@ -492,14 +255,33 @@ def main():
                     else:
                         bit_exact &= bool(np.array_equal(ref[0], o["level"]) and
                                           np.array_equal(ref[1], o["parent"]))
-    if len(policies) < 2:
-        bit_exact = None  # nothing to compare against
     log(f"warmup done, policies bit-exact: {bit_exact}")
+
+    # the per-level oracle choice: every choice timed on a snapshot of each
+    # level (the BEST policy), then those choices replayed as a script
+    scripts = None
+    if args.policy == "both":
+        t = time.time()
+        scripts = []
+        for i, r in enumerate(roots):
+            o = dg.bfs(int(r), policy=A.SWITCH_BEST, timing_reps=3, results=(i in keep))
+            scripts.append(o["choice"].astype(np.int32))
+            if i in keep:
+                bit_exact &= bool(np.array_equal(keep[i][0], o["level"]) and
+                                  np.array_equal(keep[i][1], o["parent"]))
+        for i, r in enumerate(roots):
+            o = dg.bfs(int(r), policy=A.SWITCH_SCRIPT, script=scripts[i], results=(i in keep))
+            if i in keep:
+                bit_exact &= bool(np.array_equal(keep[i][0], o["level"]) and
+                                  np.array_equal(keep[i][1], o["parent"]))
+        log(f"per-level oracle choices found ({time.time() - t:.1f}s), bit-exact: {bit_exact}")
+        policies.append(("oracle_choice", A.SWITCH_SCRIPT, None))
 
     results = {}
     for name, pol, s in policies:
         log(f"timing policy {name}")
-        per_root, step_ms, comp = run_policy(dg, roots, pol, s, args.steps)
+        per_root, step_ms, comp = run_policy(dg, roots, pol, s, args.steps,
+                                             scripts if pol == A.SWITCH_SCRIPT else None)
         gt = [A.gteps(g.nnz, ms) for ms in per_root]
         # Graph500 convention: undirected edges of the root's component (each
         # stored twice, no self loops) over the BFS time
@@ -511,7 +293,7 @@ def main():
         log(f"  {name}: harmonic-mean GTEPS (events) {results[name]['hm_gteps_events']:.2f}, "
             f"wall {results[name]['wall_gteps']:.2f}, step {step_ms:.1f} ms")
 
-    main_name = "learned" if "learned" in results else "heuristic"
+    main_name = "learned" if "learned" in results else policies[0][0]
     R = results[main_name]
     pol_main, sel_main = [(p, s) for nm, p, s in policies if nm == main_name][0]
     prof = account_kernels(dg, roots, pol_main, sel_main, g.n)
@@ -538,6 +320,11 @@ def main():
                                 "gbps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
                                 if v["ms"] > 0 else None}
                             for k, v in sorted(prof.items())}}
+
+    def ratio(a, b):
+        return round(results[a]["wall_gteps"] / results[b]["wall_gteps"], 4) \
+            if a in results and b in results else None
+
     line = {
         "metric": METRIC, "value": round(R["wall_gteps"], 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(R["step_ms"], 3),
@@ -550,12 +337,13 @@ def main():
                    "layout": "SELL-C-sigma C=64 sigma=4096 pad=1", "policy": main_name},
         "heuristic_gteps": (round(results["heuristic"]["wall_gteps"], 3)
                             if "heuristic" in results else None),
-        "learned_over_heuristic": (round(results["learned"]["wall_gteps"] /
-                                         results["heuristic"]["wall_gteps"], 4)
-                                   if len(results) == 2 else None),
+        "oracle_choice_gteps": (round(results["oracle_choice"]["wall_gteps"], 3)
+                                if "oracle_choice" in results else None),
+        "learned_over_heuristic": ratio("learned", "heuristic"),
+        "learned_over_oracle_choice": ratio("learned", "oracle_choice"),
         "hm_gteps_events": {k: round(v["hm_gteps_events"], 3) for k, v in results.items()},
         "hm_gteps_graph500": {k: round(v["hm_gteps_graph500"], 3) for k, v in results.items()},
-        "bit_exact": bit_exact,
+        "bit_exact": bit_exact if len(policies) > 1 else None,
         "pcie_inclusive_gteps": round(pcie_gteps, 3),
         "sell_build": sell_build,
         "graph_generate_ms": {k: round(v, 1) for k, v in gen_report.items()},
@@ -563,9 +351,18 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(g, roots, keep)
+    # a result that is not bit-exact carries no value and fails the run
+    ok = bit_exact is not False and line.get("cpu_baseline", {}).get("levels_match_gpu", True)
+    if not ok:
+        line["value"] = None
+        line["error"] = "levels or parents differ between policies or from the oracle"
     print(json.dumps(line), flush=True)
+    if sel is not None:
+        sel.close()
     dg.close()
     ctx.close()
+    if not ok:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

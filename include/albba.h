@@ -179,6 +179,7 @@ typedef struct albba_sellcs {
      * symmetric, where the push CSR holds the same rows */
     int64_t *pull_ptr;        /* [n+1] or NULL                              */
     int32_t *pull_col;        /* [nnz] or NULL                              */
+    int32_t order_flags;      /* ALBBA_SELL_* row-order flags it was built with */
 } albba_sellcs;
 
 typedef struct albba_ctx albba_ctx;       /* one device + one stream           */
@@ -305,6 +306,19 @@ ALBBA_API uint64_t albba_rng_u64(uint64_t seed, uint64_t idx);
  * pad in {1,16,32} entries per column.  Builds the push CSR as well. */
 ALBBA_API int albba_sellcs_build(const albba_csr *g, int C, int sigma, int pad,
                                  albba_sellcs **out);
+/* Row-order flags of albba_sellcs_build_ex / albba_graph_build_ex.
+ * ALBBA_SELL_SORT_FIRST_COL: before the sigma-window sort, order the rows
+ * by their first column index, descending and stable (the reference's
+ * `-s 1`, sellcs_preprocess(..., sort_flag = true): bfs/src/bfs_SELLCSv2.c:
+ * 40-45, sort_by_1st_colidx bfs/src/sorting_utils.c:255-281, then the window
+ * sort of get_order_by_row_size_radix_sorted :282-297).  A row's first column
+ * is its smallest in-neighbour id; an empty row takes the next stored entry
+ * (the next non-empty row's first column), 0 past the last entry. */
+enum { ALBBA_SELL_SORT_FIRST_COL = 1 };
+/* albba_sellcs_build with row-order flags (ALBBA_SELL_*); flags = 0 is
+ * albba_sellcs_build. */
+ALBBA_API int albba_sellcs_build_ex(const albba_csr *g, int C, int sigma, int pad, int flags,
+                                    albba_sellcs **out);
 ALBBA_API void albba_sellcs_free(albba_sellcs *s);
 
 /* ---- device ------------------------------------------------------------- */
@@ -330,6 +344,10 @@ typedef struct albba_build_report {
 } albba_build_report;
 ALBBA_API int albba_graph_build(albba_ctx *ctx, const albba_csr *g, int C, int sigma, int pad,
                                 albba_graph **out, albba_build_report *report);
+/* albba_graph_build with row-order flags (ALBBA_SELL_*): the same device
+ * arrays as albba_sellcs_build_ex with the same flags + albba_graph_upload. */
+ALBBA_API int albba_graph_build_ex(albba_ctx *ctx, const albba_csr *g, int C, int sigma, int pad,
+                                   int flags, albba_graph **out, albba_build_report *report);
 /* Load a .cbd file (bfs/src/biio.h:21-71 layout) straight into device
  * memory -- the row pointers and columns stream through pinned buffers, file
  * reads overlapping the copies; the values are not read -- then form the

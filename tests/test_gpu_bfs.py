@@ -23,6 +23,13 @@ POLICIES = [("heuristic", A.SWITCH_HEURISTIC, 0), ("push", A.SWITCH_FIXED, A.CHO
             ("pull_cb", A.SWITCH_FIXED, A.CHOICE_PULL_CB), ("best", A.SWITCH_BEST, 0)]
 
 
+def modes_as_choices(mode):
+    """The oracle's per-level mode (1: SpMV pull, 0: SpMSpV) as the kernel
+    choice the heuristic policy takes for it (the slice-bypass pull, or the
+    push)."""
+    return np.where(np.asarray(mode) == 1, A.CHOICE_PULL_SB, A.CHOICE_PUSH)
+
+
 def check(dg, rp, col, root, policy=A.SWITCH_HEURISTIC, fixed=0, **kw):
     o = dg.bfs(int(root), policy=policy, fixed_choice=fixed, **kw)
     lv, pa = O.bfs_queue(rp, col, int(root))
@@ -143,6 +150,8 @@ def test_config0_rmat16_sixteen_roots(ctx):
             o = check(dg, rp, col, root)
             ref = O.albba_bfs(rp, col, int(root), 256, 16384, 8)
             np.testing.assert_array_equal(o["frontier"], ref["frontier"])
+            # the heuristic's direction per level is the reference's mode
+            np.testing.assert_array_equal(o["choice"], modes_as_choices(ref["mode"]))
         dg.close()
 
 
@@ -161,10 +170,41 @@ def test_config1_rmat22_levels_bit_exact(ctx):
             ref = O.albba_bfs_on(s, g.n, int(root), tasks=threads)
             np.testing.assert_array_equal(o["level"], ref["level"])
             np.testing.assert_array_equal(o["parent"], ref["parent"])
+            np.testing.assert_array_equal(o["choice"], modes_as_choices(ref["mode"]))
             if i < 2:
                 assert O.validate(rp, col, int(root), o["level"], o["parent"]) == 0
     finally:
         O.free_sell(s)
+        dg.close()
+
+
+def test_heuristic_threshold_boundary(ctx):
+    """Frontiers of exactly (int)(0.04 n) and one more vertex: the first
+    stays a push, the second pulls (bfs/src/bfs_SELLCSv2.c:282-290, 302:
+    pull iff nnzx > threshold).  n = 1000, threshold 40; level sizes 1, 40,
+    41, 40, 41, 40, 41 and isolated vertices after them."""
+    n, sizes = 1000, [1, 40, 41, 40, 41, 40, 41]
+    rows = [[] for _ in range(n)]
+    levels, nxt = [], 0
+    for s in sizes:
+        levels.append(list(range(nxt, nxt + s)))
+        nxt += s
+    for a, b in zip(levels, levels[1:]):
+        for k, v in enumerate(b):
+            for u in {a[k % len(a)], a[(3 * k + 1) % len(a)]}:
+                rows[u].append(v)
+                rows[v].append(u)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c in sorted(r)], np.int32)
+    g = A.Graph.from_arrays(rp, col, symmetric=True)
+    ref = O.albba_bfs(rp, col, 0, 256, 16384, 4)
+    np.testing.assert_array_equal(ref["frontier"], sizes)
+    np.testing.assert_array_equal(ref["mode"], [0, 0, 1, 0, 1, 0, 1])
+    for C_ in (16, 64):
+        dg = A.DeviceGraph(ctx, g.sellcs(C_, 256, 1))
+        o = check(dg, rp, col, 0)
+        np.testing.assert_array_equal(o["choice"], modes_as_choices(ref["mode"]))
         dg.close()
 
 

@@ -17,9 +17,9 @@ GG = np.load(os.path.join(HERE, "golden", "golden_graphs.npz"))
 GOLDEN = sorted({k.split("__")[0] for k in GG.files})
 
 
-def assert_same_layout(ctx, g, C_, sigma, pad):
-    host = g.sellcs(C_, sigma, pad).arrays()
-    dg = A.DeviceGraph.build(ctx, g, C_, sigma, pad)
+def assert_same_layout(ctx, g, C_, sigma, pad, sort_first=False):
+    host = g.sellcs(C_, sigma, pad, sort_first=sort_first).arrays()
+    dg = A.DeviceGraph.build(ctx, g, C_, sigma, pad, sort_first=sort_first)
     dev = dg.layout().arrays()
     for k, v in host.items():
         if v is None:
@@ -39,6 +39,43 @@ def test_golden_graphs_every_shape(ctx, name):
                 continue
             for pad in (1, 16, 32):
                 assert_same_layout(ctx, g, C_, sigma, pad).close()
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_graphs_first_column_order(ctx, name):
+    """The reference's -s 1 row order built on the device: the host
+    builder's arrays, bit for bit; BFS on it (no sigma window holds its own
+    id range, so level/parent go out through the position table) against
+    the oracle; run_custom_bfs_test(sort_flag) gives the reference's
+    frontier counts."""
+    rp, col = GG[f"{name}__rp"], GG[f"{name}__col"]
+    g = A.Graph.from_arrays(rp, col, symmetric=not name.startswith("dir_"))
+    for C_, sigma, pad in [(64, 64, 1), (16, 32, 16), (128, 256, 1), (32, max(g.n, 32), 32)]:
+        if sigma % C_ and sigma < g.n:
+            continue
+        dg = assert_same_layout(ctx, g, C_, sigma, pad, sort_first=True)
+        for root in sorted({0, g.n // 2, g.n - 1}):
+            o = dg.bfs(root)
+            lv, pa = O.bfs_queue(rp, col, root)
+            np.testing.assert_array_equal(o["level"], lv)
+            np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+    out = A.run_custom_bfs_test(rp.size - 1, rp.astype(np.int32), col, sort_flag=True)
+    np.testing.assert_array_equal(out["frontier"], GG[f"{name}__bfs_ref_frontier"])
+
+
+def test_first_column_order_medium_graph(ctx):
+    g = A.Graph.rmat(15, 16, seed=101, perm_seed=102)
+    for C_, sigma in [(64, 4096), (128, 16384)]:
+        dg = assert_same_layout(ctx, g, C_, sigma, 1, sort_first=True)
+        for root in g.pick_roots(2, 103):
+            for pol, fc in ((A.SWITCH_HEURISTIC, 0), (A.SWITCH_FIXED, A.CHOICE_PULL_CB),
+                            (A.SWITCH_FIXED, A.CHOICE_PUSH)):
+                o = dg.bfs(int(root), policy=pol, fixed_choice=fc)
+                lv, pa = O.bfs_queue(g.row_ptr, g.col, int(root))
+                np.testing.assert_array_equal(o["level"], lv)
+                np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
 
 
 @pytest.mark.parametrize("kind", ["rmat", "er", "grid", "directed"])

@@ -205,11 +205,58 @@ def test_device_layout_is_the_reference_layout_renumbered(name, C_, pad):
         np.testing.assert_array_equal(rcol[rptr[p]:rptr[p + 1]], s["pos_of"][col[rp[v]:rp[v + 1]]])
 
 
+@pytest.mark.parametrize("name", GRAPHS)
+@pytest.mark.parametrize("C_,sigma,pad", [(128, 16384, 1), (64, 64, 16), (16, 32, 1)])
+def test_device_layout_first_column_order(name, C_, sigma, pad):
+    """ALBBA_SELL_SORT_FIRST_COL (the reference's -s 1): the row order is the
+    reference's sorted build (fixture at sigma 16384; the window sort does
+    not depend on C), and the layout is the reference-compatible sorted
+    builder's at the same C and sigma, renumbered to positions."""
+    rp, col = GG[f"{name}__rp"], GG[f"{name}__col"]
+    g = A.Graph.from_arrays(rp, col, symmetric=not name.startswith("dir_"))
+    n = rp.size - 1
+    s = g.sellcs(C_, sigma, pad, sort_first=True).arrays()
+    assert s["order_flags"] == A.SELL_SORT_FIRST_COL
+    ref = A.sellcs_preprocess(n, rp.astype(np.int32), col, C_, sigma, sort_flag=True)
+    if sigma == 16384:
+        np.testing.assert_array_equal(s["row_order"], GG[f"{name}__sellsorted_256_16384__row_order"])
+    np.testing.assert_array_equal(s["row_order"], ref["row_order"])
+    np.testing.assert_array_equal(s["pos_of"][s["row_order"]], np.arange(n))
+    np.testing.assert_array_equal(s["slice_widths"], ref["slice_widths"])
+    np.testing.assert_array_equal(s["vop_pointers"], ref["vop_pointers"])
+    np.testing.assert_array_equal(s["vop_lengths"], ref["vop_lengths"])
+    ci = s["column_indices"]
+    out, at = [], 0
+    for act in s["vop_lengths"].astype(np.int64) + 1:
+        out.append(ci[at:at + act])
+        at += -(-act // pad) * pad
+    flat = np.concatenate(out) if out else np.zeros(0, np.int32)
+    np.testing.assert_array_equal(flat, s["pos_of"][ref["column_indices"]])
+    # without the flag the order differs on these graphs (or is the identity
+    # on the ones whose first columns already descend)
+    plain = g.sellcs(C_, sigma, pad).arrays()
+    assert plain["order_flags"] == 0
+
+
+def test_first_column_order_differs_from_plain_order():
+    g = A.Graph.rmat(10, 8, seed=5, perm_seed=6)
+    a = g.sellcs(64, 256, 1, sort_first=True).arrays()["row_order"]
+    b = g.sellcs(64, 256, 1).arrays()["row_order"]
+    assert not np.array_equal(a, b)
+    # the window sort keeps each window's rows in first-column order on ties
+    first = np.array([g.col[g.row_ptr[v]] if g.row_ptr[v] < g.nnz else 0 for v in range(g.n)])
+    deg = np.diff(g.row_ptr)
+    key = [(-deg[v], -first[v], v) for v in a[:256]]
+    assert key == sorted(key)
+
+
 def test_device_layout_rejects_bad_parameters():
     g = A.Graph.rmat(8, 8)
     for C_, sigma, pad in [(48, 96, 1), (64, 100, 1), (64, 4096, 8), (0, 64, 1)]:
         with pytest.raises(A.AlbbaError):
             g.sellcs(C_, sigma, pad)
+    p = C.POINTER(A._SELL)()
+    assert A.lib().albba_sellcs_build_ex(C.byref(g._raw), 64, 64, 1, 2, C.byref(p)) == -1
 
 
 def test_unsorted_rows_are_sorted_for_the_layout():
