@@ -544,7 +544,7 @@ class DeviceGraph:
             small_frontier_edges: int = None, small_frontier_blocks: int = 0) -> dict:
         """Run one BFS on the device; returns stats (+ level/parent arrays).
 
-        small_frontier_edges: edge cap of the cooperative small-frontier push
+        small_frontier_edges: edge cap of the persistent small-frontier push
         levels (0 disables them; None keeps the library default);
         small_frontier_blocks: its workgroup count (0: default)."""
         o = _Opts()
@@ -604,7 +604,7 @@ MAP_LANE_ROW, MAP_WAVE_ROW = 0, 1
 
 class Batch:
     """Batched BFS: one instance per entry of `graphs` (device graphs; one
-    may appear several times), all run in one cooperative launch."""
+    may appear several times), all run in one persistent launch."""
 
     def __init__(self, ctx: Context, graphs):
         self.ctx = ctx

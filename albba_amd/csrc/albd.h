@@ -289,7 +289,7 @@ ALBD_HIDDEN int albd_push_expand(albd_ctx *c, const albd_graph *g, const albd_st
  * (ctr->frontier_edges) are at most build_max; else it stays empty */
 ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                    int32_t level, int64_t build_max);
-/* Small-frontier push levels in one cooperative kernel of `blocks`
+/* Small-frontier push levels in one persistent kernel of `blocks`
  * workgroups (0: the default; capped at what can be co-resident): starting from the
  * queue (st.queue, ctr->queue_len), run push levels level0, level0+1, ...
  * while the new frontier has at most edge_cap out-edges and vertex_cap
@@ -300,10 +300,6 @@ ALBD_HIDDEN int albd_push_finalize(albd_ctx *c, const albd_graph *g, const albd_
  * ctr->persist_levels levels ran, st.level_log holds their sizes.  A grid
  * barrier that waits more than spin_limit polls sets ctr->persist_error and
  * the kernel ends early (the keys in st.pkey must then be refilled). */
-/* albd_push_small's return when cooperative launches are unavailable on this
- * context (another process may hold the device's cooperative resources); no
- * kernel ran and the caller continues the level with the plain push */
-#define ALBD_NOCOOP 1
 ALBD_HIDDEN int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st,
                                 int32_t level0, int64_t edge_cap, int64_t vertex_cap,
                                 int32_t max_levels, int32_t blocks, uint32_t spin_limit);
@@ -362,7 +358,8 @@ typedef struct albd_batch_args {
 
 /* workgroups that can be resident at once for albd_batch_launch */
 ALBD_HIDDEN int albd_batch_capacity(albd_ctx *c, int *workgroups);
-/* one cooperative launch of `workgroups` workgroups over all teams */
+/* one launch of `workgroups` co-resident workgroups (at most
+ * albd_batch_capacity) over all teams */
 ALBD_HIDDEN int albd_batch_launch(albd_ctx *c, const albd_batch_args *a, int32_t workgroups);
 /* ticks per millisecond of the device clock read by the batch kernel */
 ALBD_HIDDEN double albd_clock_ticks_per_ms(albd_ctx *c);

@@ -27,7 +27,6 @@ int albd_set_error_msg(int code, const char *msg)
 struct albd_ctx {
     int device;
     int ncu;
-    int coop_off;
     hipStream_t stream;
     hipEvent_t ev[ALBD_NEVENTS];
     albd_counters *hctr; // pinned
@@ -89,8 +88,6 @@ void albd_ctx_destroy(albd_ctx *c)
 int albd_ctx_device(const albd_ctx *c) { return c->device; }
 int albd_num_cus(const albd_ctx *c) { return c->ncu; }
 hipStream_t albd_stream(albd_ctx *c) { return c->stream; }
-int albd_coop_off(const albd_ctx *c) { return c->coop_off; }
-void albd_set_coop_off(albd_ctx *c) { c->coop_off = 1; }
 
 int albd_malloc(albd_ctx *c, size_t bytes, void **p)
 {

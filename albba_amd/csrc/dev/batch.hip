@@ -1,5 +1,5 @@
 // Batched BFS: many instances (graphs and roots of uneven size) in one
-// cooperative launch.  The workgroups are split into teams; a team runs the
+// persistent launch of co-resident workgroups.  The workgroups are split into teams; a team runs the
 // complete level loop of its instances one after the other, with one team
 // barrier per level and the direction of every level chosen on the device
 // (the reference heuristic, bfs/src/bfs_SELLCSv2.c:282-284, or the learned
@@ -429,10 +429,16 @@ int albd_batch_capacity(albd_ctx *c, int *workgroups)
 
 int albd_batch_launch(albd_ctx *c, const albd_batch_args *a, int32_t workgroups)
 {
-    albd_batch_args args = *a;
-    void *p[] = {&args};
-    HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_bfs_batch), dim3(workgroups),
-                                      dim3(kBB), p, 0, albd_stream(c)));
+    // at most albd_batch_capacity workgroups, all resident at once; an
+    // ordinary launch on the context's stream, which starts once the
+    // stream's earlier kernels have finished (a team barrier that waited in
+    // vain would give up after spin_limit polls, never hang)
+    int cap = 0;
+    if (int rc = albd_batch_capacity(c, &cap)) return rc;
+    if (workgroups < 1 || workgroups > cap)
+        return albd_set_error_msg(ALBBA_EINVAL, "batch launch: more workgroups than can be resident");
+    hipLaunchKernelGGL(k_bfs_batch, dim3(workgroups), dim3(kBB), 0, albd_stream(c), *a);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

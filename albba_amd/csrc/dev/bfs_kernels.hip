@@ -1042,12 +1042,12 @@ __global__ __launch_bounds__(kBlock) void k_push_expand(albd_graph g, albd_state
 }
 
 // ---------------------------------------------------------------------------
-// small-frontier push levels in one cooperative launch
+// small-frontier push levels in one persistent launch
 //
 // High-diameter graphs spend thousands of levels with a few thousand
 // frontier vertices; a launch sequence and a host round trip per level would
 // dominate, and one workgroup alone is bound by its CU's gather rate.  A
-// cooperative grid of G co-resident workgroups runs the push levels back to
+// grid of G co-resident workgroups runs the push levels back to
 // back with one grid barrier per level, until the next frontier has more
 // than edge_cap out-edges or more than vertex_cap vertices.
 //
@@ -1564,26 +1564,23 @@ int albd_push_small(albd_ctx *c, const albd_graph *g, const albd_state *st, int3
 {
     if (max_levels > ALBD_PERSIST_LEVELS) max_levels = ALBD_PERSIST_LEVELS;
     if (max_levels < 1) max_levels = 1;
-    if (albd_coop_off(c)) return ALBD_NOCOOP;
-    // every workgroup of the grid barrier must be resident at once
-    static int per_cu = -1; // one gfx950 code object: the same on every device
-    if (per_cu < 0)
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
+    // Every workgroup of the grid barrier must be resident at once: the grid
+    // is at most what the occupancy allows on every CU together, and the
+    // launch is an ordinary one on the context's stream, whose earlier
+    // kernels have finished before it starts (stream order).  A workgroup
+    // that could not become resident would only make the barrier give up
+    // after spin_limit polls (ctr->persist_error), never hang.
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push_small, kSmallBlock, 0));
     const int cap = per_cu * albd_num_cus(c);
     if (blocks <= 0) blocks = 64;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) return albd_set_error_msg(ALBBA_EHIP, "small-frontier kernel cannot be resident");
-    hipStream_t s = albd_stream(c);
     // level 0's counters, the grid state and the error flag were zeroed by
     // the level-begin kernel (LB_SMALL)
     SmallArgs a{*g, *st, level0, max_levels, edge_cap, vertex_cap, spin_limit};
-    void *args[] = {&a};
-    if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_push_small), dim3(blocks),
-                                   dim3(kSmallBlock), args, 0, s) != hipSuccess) {
-        (void)hipGetLastError(); // nothing was launched; clear the error state
-        albd_set_coop_off(c);
-        return ALBD_NOCOOP;
-    }
+    hipLaunchKernelGGL(k_push_small, dim3(blocks), dim3(kSmallBlock), 0, albd_stream(c), a);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

@@ -7,9 +7,6 @@
 int albd_set_error(hipError_t e, const char *what, const char *file, int line);
 int albd_set_error_msg(int code, const char *msg);
 hipStream_t albd_stream(albd_ctx *c);
-// cooperative launches failed once on this context: do not try again
-int albd_coop_off(const albd_ctx *c);
-void albd_set_coop_off(albd_ctx *c);
 
 #define HIPCHK(x)                                                          \
     do {                                                                   \

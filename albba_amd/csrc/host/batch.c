@@ -1,6 +1,6 @@
 /*
  * Batched BFS host side: per-instance device scratch, the split of the
- * resident workgroups into teams, one cooperative launch, per-instance
+ * resident workgroups into teams, one persistent launch, per-instance
  * results from the device clock.
  *
  * Team split (split_teams): an instance's work is its stored entries plus
@@ -17,15 +17,29 @@
 struct albba_batch {
     albba_ctx *ctx;
     int32_t count;
+    int32_t wcap;                 /* resident workgroups of the batch kernel */
     albba_graph **graphs;
-    albd_batch_item *items;       /* host copies */
+    albd_batch_item *items;       /* host copies (pinned, head of `stage`) */
     unsigned long long *ctl;      /* device: count * CTL_WORDS */
-    unsigned long long *hctl;     /* host mirror */
+    unsigned long long *hctl;     /* host mirror (pinned) */
     void **allocs;                /* device allocations to free */
     int32_t nallocs;
     int32_t *team_of;             /* instance -> team, team sizes of the last run */
     int32_t *team_size_of;
+    /* host work arrays of a run, sized at creation for `wcap` workgroups, so
+     * that a run allocates no host memory */
+    int32_t *order, *share, *cnt, *team_first, *team_size, *team_ptr, *team_items, *wg_team;
+    double *load;
+    /* pinned staging of a run's device tables: the instance descriptors, then
+     * (at tab_off bytes) the team tables */
+    void *stage;
+    size_t tab_off, stage_bytes;
 };
+
+/* device bytes of a run's tables: instance descriptors, then the team tables
+ * (W workgroups, T teams) at a 256-byte boundary */
+static size_t tab_words(int32_t N, int32_t W, int32_t T) { return (size_t)W + 3 * (size_t)T + 1 + (size_t)N; }
+static size_t tab_offset(int32_t N) { return (sizeof(albd_batch_item) * (size_t)N + 255) / 256 * 256; }
 
 static void *bdalloc(albba_batch *b, size_t bytes, int *rc)
 {
@@ -40,12 +54,14 @@ void albba_batch_free(albba_batch *b)
 {
     if (!b) return;
     for (int32_t i = 0; i < b->nallocs; ++i) albd_free(b->ctx->d, b->allocs[i]);
+    albd_pinned_free(b->ctx->d, b->stage);
+    albd_pinned_free(b->ctx->d, b->hctl);
     free(b->allocs);
     free(b->graphs);
-    free(b->items);
-    free(b->hctl);
     free(b->team_of);
     free(b->team_size_of);
+    free(b->order); free(b->share); free(b->cnt); free(b->team_first); free(b->team_size);
+    free(b->team_ptr); free(b->team_items); free(b->wg_team); free(b->load);
     free(b);
 }
 
@@ -55,21 +71,46 @@ int albba_batch_create(albba_ctx *ctx, albba_graph *const *graphs, int32_t count
     for (int32_t i = 0; i < count; ++i)
         if (!graphs[i] || graphs[i]->ctx != ctx)
             return albba_fail(ALBBA_EINVAL, "batch_create: graph %d missing or on another context", i);
+    int wcap = 0;
+    int rc = albd_batch_capacity(ctx->d, &wcap);
+    if (rc) return albba_fail(rc, "batch_create: %s", albd_error());
+    if (wcap < 1) return albba_fail(ALBBA_EHIP, "batch_create: no resident workgroup");
     albba_batch *b = (albba_batch *)albba_calloc(1, sizeof(albba_batch));
     if (!b) return albba_fail(ALBBA_ENOMEM, "batch_create");
     b->ctx = ctx;
     b->count = count;
-    b->graphs = (albba_graph **)albba_calloc((size_t)count, sizeof(albba_graph *));
-    b->items = (albd_batch_item *)albba_calloc((size_t)count, sizeof(albd_batch_item));
-    b->hctl = (unsigned long long *)albba_calloc((size_t)count * CTL_WORDS, sizeof(unsigned long long));
-    b->allocs = (void **)albba_calloc((size_t)count * 12 + 1, sizeof(void *));
-    b->team_of = (int32_t *)albba_calloc((size_t)count, sizeof(int32_t));
-    b->team_size_of = (int32_t *)albba_calloc((size_t)count, sizeof(int32_t));
-    if (!b->graphs || !b->items || !b->hctl || !b->allocs || !b->team_of || !b->team_size_of) {
+    b->wcap = wcap;
+    const size_t N = (size_t)count, W = (size_t)wcap, T = N < W ? N : W;
+    b->graphs = (albba_graph **)albba_calloc(N, sizeof(albba_graph *));
+    b->allocs = (void **)albba_calloc(N * 12 + 1, sizeof(void *));
+    b->team_of = (int32_t *)albba_calloc(N, sizeof(int32_t));
+    b->team_size_of = (int32_t *)albba_calloc(N, sizeof(int32_t));
+    b->order = (int32_t *)albba_calloc(N, sizeof(int32_t));
+    b->share = (int32_t *)albba_calloc(N, sizeof(int32_t));
+    b->cnt = (int32_t *)albba_calloc(W + 1, sizeof(int32_t));
+    b->load = (double *)albba_calloc(W, sizeof(double));
+    b->team_first = (int32_t *)albba_calloc(T, sizeof(int32_t));
+    b->team_size = (int32_t *)albba_calloc(T, sizeof(int32_t));
+    b->team_ptr = (int32_t *)albba_calloc(T + 1, sizeof(int32_t));
+    b->team_items = (int32_t *)albba_calloc(N, sizeof(int32_t));
+    b->wg_team = (int32_t *)albba_calloc(W, sizeof(int32_t));
+    b->tab_off = tab_offset(count);
+    b->stage_bytes = b->tab_off + sizeof(int32_t) * tab_words(count, wcap, (int32_t)T);
+    if (!b->graphs || !b->allocs || !b->team_of || !b->team_size_of || !b->order || !b->share ||
+        !b->cnt || !b->load || !b->team_first || !b->team_size || !b->team_ptr || !b->team_items ||
+        !b->wg_team) {
         albba_batch_free(b);
         return albba_fail(ALBBA_ENOMEM, "batch_create");
     }
-    int rc = 0;
+    if ((rc = albd_pinned_alloc(ctx->d, b->stage_bytes, &b->stage)) ||
+        (rc = albd_pinned_alloc(ctx->d, sizeof(unsigned long long) * CTL_WORDS * N, (void **)&b->hctl))) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s", albd_error());
+        albba_batch_free(b);
+        return albba_fail(rc, "batch_create: %s", buf);
+    }
+    b->items = (albd_batch_item *)b->stage;
+    memset(b->stage, 0, b->stage_bytes);
     b->ctl = (unsigned long long *)bdalloc(b, sizeof(unsigned long long) * CTL_WORDS * (size_t)count, &rc);
     for (int32_t i = 0; i < count && !rc; ++i) {
         albba_graph *G = graphs[i];
@@ -108,21 +149,17 @@ static double inst_cost(const albba_graph *G) { return (double)G->g.nnz + (doubl
  * least 2 workgroups get a team of their own of that size (at most 64, and
  * at most half of W together); the other instances go, largest first, to the
  * least loaded of the remaining one-workgroup teams. */
-static int32_t split_teams(const albba_batch *b, int32_t W, int32_t *team_first, int32_t *team_size,
+static int32_t split_teams(albba_batch *b, int32_t W, int32_t *team_first, int32_t *team_size,
                            int32_t *team_ptr, int32_t *team_items, int32_t *wg_team, int32_t *inst_team)
 {
     const int32_t N = b->count;
     double total = 0;
     for (int32_t i = 0; i < N; ++i) total += inst_cost(b->graphs[i]);
     if (total <= 0) total = 1;
-    int32_t *order = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
-    int32_t *share = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
-    double *load = (double *)albba_calloc((size_t)W, sizeof(double));
-    int32_t *cnt = (int32_t *)albba_calloc((size_t)W + 1, sizeof(int32_t));
-    if (!order || !share || !load || !cnt) {
-        free(order); free(share); free(load); free(cnt);
-        return -1;
-    }
+    int32_t *order = b->order, *share = b->share, *cnt = b->cnt;
+    double *load = b->load;
+    memset(load, 0, sizeof(double) * (size_t)W);
+    memset(cnt, 0, sizeof(int32_t) * ((size_t)W + 1));
     for (int32_t i = 0; i < N; ++i) {
         order[i] = i;
         double s = (double)W * inst_cost(b->graphs[i]) / total;
@@ -179,98 +216,82 @@ static int32_t split_teams(const albba_batch *b, int32_t W, int32_t *team_first,
         team_first[t] = wg;
         for (int32_t s = 0; s < team_size[t]; ++s) wg_team[wg++] = t;
     }
-    free(order); free(share); free(load); free(cnt);
     return T;
 }
 
-int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
-                    const int32_t *mapping, int32_t workgroups, albba_batch_result *results,
-                    float *total_ms)
+/* One run.  The device tables (instance descriptors, team tables) go to
+ * fresh device memory; the host side uses the batch's own arrays. */
+static int batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts, const int32_t *mapping,
+                     int32_t workgroups, albba_batch_result *results, float *total_ms)
 {
-    if (!b || !roots) return albba_fail(ALBBA_EINVAL, "batch_run: bad argument");
+    const char *what = "batch_run";
+    if (!b || !roots) return albba_fail(ALBBA_EINVAL, "%s: bad argument", what);
     albba_bfs_opts o;
     if (opts) o = *opts;
     else albba_bfs_opts_default(&o);
     if (o.switch_policy != ALBBA_SWITCH_HEURISTIC && o.switch_policy != ALBBA_SWITCH_LEARNED)
-        return albba_fail(ALBBA_EINVAL, "batch_run: policy must be heuristic or learned");
+        return albba_fail(ALBBA_EINVAL, "%s: policy must be heuristic or learned", what);
     if (o.switch_policy == ALBBA_SWITCH_LEARNED && !o.selector)
-        return albba_fail(ALBBA_EINVAL, "batch_run: learned policy needs a selector");
+        return albba_fail(ALBBA_EINVAL, "%s: learned policy needs a selector", what);
     const int32_t N = b->count;
     for (int32_t i = 0; i < N; ++i) {
         if (roots[i] < 0 || roots[i] >= b->graphs[i]->n)
-            return albba_fail(ALBBA_EINVAL, "batch_run: root %d of instance %d out of range", roots[i], i);
+            return albba_fail(ALBBA_EINVAL, "%s: root %d of instance %d out of range", what, roots[i], i);
         if (mapping && mapping[i] != ALBBA_MAP_LANE_ROW && mapping[i] != ALBBA_MAP_WAVE_ROW)
-            return albba_fail(ALBBA_EINVAL, "batch_run: mapping %d of instance %d", mapping[i], i);
+            return albba_fail(ALBBA_EINVAL, "%s: mapping %d of instance %d", what, mapping[i], i);
     }
     albd_ctx *d = b->ctx->d;
-    int cap = 0, rc = albd_batch_capacity(d, &cap);
-    if (rc) return albba_fail(rc, "batch_run: %s", albd_error());
-    int32_t W = workgroups > 0 && workgroups < cap ? workgroups : cap;
-    if (W < 1) return albba_fail(ALBBA_EHIP, "batch_run: no resident workgroup");
-    const int32_t Tmax = N < W ? N : W;
-    int32_t *team_first = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)Tmax);
-    int32_t *team_size = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)Tmax);
-    int32_t *team_ptr = (int32_t *)albba_malloc(sizeof(int32_t) * ((size_t)Tmax + 1));
-    int32_t *team_items = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)N);
-    int32_t *wg_team = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)W);
+    const int32_t W = workgroups > 0 && workgroups < b->wcap ? workgroups : b->wcap;
     int32_t *inst_team = b->team_of;
-    if (!team_first || !team_size || !team_ptr || !team_items || !wg_team) {
-        free(team_first); free(team_size); free(team_ptr); free(team_items); free(wg_team);
-        return albba_fail(ALBBA_ENOMEM, "batch_run");
-    }
-    const int32_t T = split_teams(b, W, team_first, team_size, team_ptr, team_items, wg_team, inst_team);
+    const int32_t T = split_teams(b, W, b->team_first, b->team_size, b->team_ptr, b->team_items, b->wg_team,
+                                  inst_team);
+    if (T <= 0) return albba_fail(ALBBA_EHIP, "%s: no team", what);
     int32_t used = 0;
-    for (int32_t t = 0; t < T; ++t) used += team_size[t];
-    for (int32_t i = 0; i < N && T > 0; ++i) {
+    for (int32_t t = 0; t < T; ++t) used += b->team_size[t];
+    for (int32_t i = 0; i < N; ++i) {
         b->items[i].root = roots[i];
         b->items[i].mapping = mapping ? mapping[i] : ALBD_MAP_LANE_ROW;
-        b->team_size_of[i] = team_size[inst_team[i]];
+        b->team_size_of[i] = b->team_size[inst_team[i]];
     }
+    const size_t words = tab_words(N, W, T), bytes = b->tab_off + sizeof(int32_t) * words;
     /* device copies of the tables */
-    void *d_items = NULL, *d_tab = NULL;
-    const size_t tab_words = (size_t)W + 3 * (size_t)T + 1 + (size_t)N;
-    if (T <= 0) rc = ALBBA_ENOMEM;
-    if (!rc) rc = albd_malloc(d, sizeof(albd_batch_item) * (size_t)N, &d_items);
-    if (!rc) rc = albd_malloc(d, sizeof(int32_t) * tab_words, &d_tab);
-    int32_t *h_tab = rc ? NULL : (int32_t *)albba_malloc(sizeof(int32_t) * tab_words);
-    if (!rc && !h_tab) rc = ALBBA_ENOMEM;
-    float ms = 0;
-    if (!rc) {
-        int32_t *p = h_tab;
-        memcpy(p, wg_team, sizeof(int32_t) * (size_t)used);
-        p += W;
-        memcpy(p, team_first, sizeof(int32_t) * (size_t)T);
-        p += T;
-        memcpy(p, team_size, sizeof(int32_t) * (size_t)T);
-        p += T;
-        memcpy(p, team_ptr, sizeof(int32_t) * ((size_t)T + 1));
-        p += T + 1;
-        memcpy(p, team_items, sizeof(int32_t) * (size_t)N);
-        albd_batch_args a;
-        memset(&a, 0, sizeof a);
-        const int32_t *dt = (const int32_t *)d_tab;
-        a.items = (const albd_batch_item *)d_items;
-        a.wg_team = dt;
-        a.team_first = dt + W;
-        a.team_size = dt + W + T;
-        a.team_ptr = dt + W + 2 * T;
-        a.team_items = dt + W + 3 * T + 1;
-        a.params = o.switch_policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(o.selector) : NULL;
-        a.spin_limit = 1u << 24;
-        if ((rc = albd_h2d(d, d_items, b->items, sizeof(albd_batch_item) * (size_t)N)) ||
-            (rc = albd_h2d(d, d_tab, h_tab, sizeof(int32_t) * tab_words)) ||
-            (rc = albd_fill_u64(d, b->ctl, 0, (size_t)N * CTL_WORDS)) || (rc = albd_event_record(d, 4)) ||
-            (rc = albd_batch_launch(d, &a, used)) || (rc = albd_event_record(d, 5)) ||
-            (rc = albd_event_elapsed(d, 4, 5, &ms)) ||
-            (rc = albd_d2h(d, b->hctl, b->ctl, sizeof(unsigned long long) * (size_t)N * CTL_WORDS))) {
-        }
+    void *dev = NULL;
+    int rc = 0;
+    if ((rc = albd_malloc(d, bytes, &dev))) {
+        return albba_fail(rc, "%s: %s", what, albd_error());
     }
+    int32_t *h_tab = (int32_t *)((char *)b->stage + b->tab_off), *p = h_tab;
+    memcpy(p, b->wg_team, sizeof(int32_t) * (size_t)used);
+    p += W;
+    memcpy(p, b->team_first, sizeof(int32_t) * (size_t)T);
+    p += T;
+    memcpy(p, b->team_size, sizeof(int32_t) * (size_t)T);
+    p += T;
+    memcpy(p, b->team_ptr, sizeof(int32_t) * ((size_t)T + 1));
+    p += T + 1;
+    memcpy(p, b->team_items, sizeof(int32_t) * (size_t)N);
+    albd_batch_args a;
+    memset(&a, 0, sizeof a);
+    const int32_t *dt = (const int32_t *)((char *)dev + b->tab_off);
+    a.items = (const albd_batch_item *)dev;
+    a.wg_team = dt;
+    a.team_first = dt + W;
+    a.team_size = dt + W + T;
+    a.team_ptr = dt + W + 2 * T;
+    a.team_items = dt + W + 3 * T + 1;
+    a.params = o.switch_policy == ALBBA_SWITCH_LEARNED ? albba_selector_dparams(o.selector) : NULL;
+    a.spin_limit = 1u << 24;
+    float ms = 0;
+    /* descriptors and tables in one copy from the pinned staging buffer */
+    (void)((rc = albd_h2d(d, dev, b->stage, bytes)) ||
+           (rc = albd_fill_u64(d, b->ctl, 0, (size_t)N * CTL_WORDS)) || (rc = albd_event_record(d, 4)) ||
+           (rc = albd_batch_launch(d, &a, used)) || (rc = albd_event_record(d, 5)) ||
+           (rc = albd_event_elapsed(d, 4, 5, &ms)) ||
+           (rc = albd_d2h(d, b->hctl, b->ctl, sizeof(unsigned long long) * (size_t)N * CTL_WORDS)));
     char buf[256] = "";
-    if (rc) snprintf(buf, sizeof buf, "%s", rc == ALBBA_ENOMEM ? "out of host memory" : albd_error());
-    albd_free(d, d_items);
-    albd_free(d, d_tab);
-    free(h_tab); free(team_first); free(team_size); free(team_ptr); free(team_items); free(wg_team);
-    if (rc) return albba_fail(rc, "batch_run: %s", buf);
+    if (rc) snprintf(buf, sizeof buf, "%s", albd_error());
+    albd_free(d, dev);
+    if (rc) return albba_fail(rc, "%s: %s", what, buf);
     const double tpm = albd_clock_ticks_per_ms(d);
     long long t_first = 0;
     int any_error = 0;
@@ -304,9 +325,16 @@ int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *
         for (int32_t i = 0; i < N; ++i)
             albd_fill_u64(d, b->items[i].key, ~0ull, (size_t)b->items[i].g.npos);
         albd_sync(d);
-        return albba_fail(ALBBA_EHIP, "batch_run: a team barrier timed out");
+        return albba_fail(ALBBA_EHIP, "%s: a team barrier timed out", what);
     }
     return ALBBA_OK;
+}
+
+int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
+                    const int32_t *mapping, int32_t workgroups, albba_batch_result *results,
+                    float *total_ms)
+{
+    return batch_run(b, roots, opts, mapping, workgroups, results, total_ms);
 }
 
 int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent)

@@ -846,15 +846,6 @@ static int run_small(albba_graph *G, const loop_state *ls, int32_t level, int64_
     if (mark && (rc = albd_event_record(d, 6))) return rc;
     rc = albd_push_small(d, &G->g, &G->st, level, edge_cap, vertex_cap, ALBD_PERSIST_LEVELS,
                          blocks, 1u << 22);
-    if (rc == ALBD_NOCOOP) {
-        /* no cooperative launch on this device right now: the level runs as
-         * a plain push from the queue already built */
-        *nlev = 1;
-        if ((rc = albd_push_scan(d, &G->g, &G->st, frontier)) ||
-            (rc = albd_push_expand(d, &G->g, &G->st)) || (mark && (rc = albd_event_record(d, 7))))
-            return rc;
-        return albd_push_finalize(d, &G->g, &G->st, level, INT64_MAX);
-    }
     if (rc) return rc;
     if (mark && (rc = albd_event_record(d, 7))) return rc;
     if ((rc = albd_read_counters(d, &G->st))) return rc;
@@ -1073,10 +1064,10 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                 return albba_fail(rc, "bfs: %s", albd_error());
         }
         /* A pushed level with a small frontier, after two more such levels,
-         * starts a run of push levels in the cooperative kernel; it continues
+         * starts a run of push levels in the persistent kernel; it continues
          * while the frontier stays within the edge cap and within the
          * heuristic's push threshold (bfs/src/bfs_SELLCSv2.c:282-284), so the
-         * heuristic's choices are unchanged.  The cooperative launch costs
+         * heuristic's choices are unchanged.  The persistent launch costs
          * more than one level: it pays off on slowly growing frontiers
          * (high-diameter graphs), which the streak detects, not on the one or
          * two small levels of a power-law BFS. */

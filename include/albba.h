@@ -216,7 +216,7 @@ typedef struct albba_bfs_opts {
     int32_t timing_reps;         /* ALBBA_SWITCH_BEST / sampling: reps per choice  */
     /* Heuristic and learned policies: when a level pushes and its frontier has
      * at most this many out-edges, after two such levels in a row (a slowly
-     * growing frontier), the following levels run inside one cooperative push
+     * growing frontier), the following levels run inside one persistent push
      * kernel (one grid barrier per level, no host round trip) until the
      * frontier outgrows the cap or the heuristic's push threshold (0
      * disables; default 65536).  Results are identical either way. */
@@ -390,7 +390,7 @@ ALBBA_API int albba_bfs_results(albba_ctx *ctx, albba_graph *g, int32_t *level,
                                 int32_t *parent);
 
 /* ---- batched BFS ---------------------------------------------------------
- * Many BFS instances (graphs and roots of uneven size) in one cooperative
+ * Many BFS instances (graphs and roots of uneven size) in one persistent
  * launch: the resident workgroups are split into teams, each team runs the
  * whole level loop of its instances with the direction of every level chosen
  * on the device.  No counterpart in the reference, which runs one BFS per
