@@ -235,6 +235,361 @@ def run_batch_workload(args):
         policies.append(("learned", A.SWITCH_LEARNED, sel))
     if args.policy != "both":
         policies = [p for p in policies if p[0] == args.policy]
+    t = time.time()
+    tuned = W.tune_layouts(ctx, specs, log=log)
+    log(f"autotuning {time.time() - t:.1f}s")
+    t = time.time()
+    dgs, roots, nnz, kinds, keep = [], [], [], [], {}
+    for i, (kind, p) in enumerate(specs):
+        cfg = W.tuned_config(tuned, kind, p)
+        dg, g = W.generate_instance(ctx, kind, p, cfg)
+        dgs.append(dg)
+        roots.append(W.root_of(kind, g, i))
+        nnz.append(g.nnz)
+        kinds.append(kind)
+        del g
+        if i < 3 or i % 64 == 0:
+            keep[i] = dg  # checked against the single-BFS path below
+        if (i + 1) % 32 == 0:
+            log(f"  {i + 1}/{count} instances generated ({time.time() - t:.1f}s)")
+    mapping = [W.tuned_config(tuned, k, p)["mapping"] for k, p in specs]
+    b = A.Batch(ctx, dgs)
+    total = float(np.sum(nnz))
+    res, exact = {}, True
+    for name, pol, sl in policies:
+        b.run(roots, policy=pol, selector=sl, mapping=mapping)  # warm-up
+        for i, dg in keep.items():
+            lv, pa = b.results(i)
+            o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
+            exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
+        runs = []
+        for s in range(args.steps):
+            out = b.run(roots, policy=pol, selector=sl, mapping=mapping)
+            runs.append(out)
+            log(f"  {name} step {s}: {out['total_ms']:.2f} ms")
+        ms = float(np.mean([o["total_ms"] for o in runs]))
+        done = np.concatenate([o["done_ms"] for o in runs])
+        serv = np.concatenate([o["ms"] for o in runs])
+        by_kind = {k: round(float(np.mean([runs[-1]["ms"][i] for i in range(count) if kinds[i] == k])), 3)
+                   for k in sorted(set(kinds))}
+        res[name] = {"gteps": round(total / (ms * 1e6), 3), "ms": round(ms, 3),
+                     "latency_ms": {"p50": round(float(np.percentile(done, 50)), 3),
+                                    "p99": round(float(np.percentile(done, 99)), 3),
+                                    "service_p50": round(float(np.percentile(serv, 50)), 3),
+                                    "service_p99": round(float(np.percentile(serv, 99)), 3)},
+                     "mean_service_ms_by_kind": by_kind,
+                     "levels_max": int(max(o["num_levels"].max() for o in runs)),
+                     "pull_levels": int(runs[-1]["pull_levels"].sum()),
+                     "push_levels": int(runs[-1]["push_levels"].sum())}
+    log(f"sampled instances bit-exact: {exact}")
+    main_name = "learned" if "learned" in res else policies[0][0]
+    R = res[main_name]
+    line = {
+        "metric": f"batched BFS total GTEPS, {count} instances (RMAT s14-20, ER 2^16-2^20, "
+                  "grids 256^2-2048^2), one launch",
+        "value": R["gteps"] if exact else None, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": R["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded mix (splitmix64 counter RNG), one root per instance",
+        "config": {"workload": f"batch{count}_mix{W.BATCH_MIX_SEED}", "instances": count,
+                   "rmat": kinds.count("rmat"), "er": kinds.count("er"), "grid": kinds.count("grid"),
+                   "nnz_stored_total": int(total), "policy": main_name,
+                   "layouts": {f"{k[0]}_2^{k[1]}": v for k, v in tuned.items()}},
+        "latency_ms": R["latency_ms"],
+        "policies": res,
+        "learned_over_heuristic": (round(res["learned"]["gteps"] / res["heuristic"]["gteps"], 4)
+                                   if len(res) == 2 else None),
+        "sampled_bit_exact": exact,
+    }
+    print(json.dumps(line), flush=True)
+    b.close()
+    for dg in dgs:
+        dg.close()
+    if sel is not None:
+        sel.close()
+    ctx.close()
+    if not exact:
+        sys.exit(1)
+
+
+def run_cpu_workload(args):
+    """The CPU baseline of the other configurations (the oracle's C + OpenMP
+    restatement of ALBBA, reference layout C=256 sigma=16384, the reference's
+    8 tasks and tasks = threads, median of 3 runs per root, harmonic mean),
+    each on a bounded sample stated in its entry; plus the dense
+    run_bfs_ref-style oracle (bfs/src/bfs_ref.h:26-67) on RMAT s16."""
+    from oracle import oracle as O
+    import albba_amd as A
+    from albba_amd import workloads as W
+
+    threads = host_threads()
+    res = {}
+    t = time.time()
+    # config 0: RMAT s16, ef 16, 16 roots (root seed 3)
+    g = A.Graph.rmat(16, 16, 0.57, 0.19, 0.19, seed=1, perm_seed=2)
+    roots = g.pick_roots(16, 3)
+    var, k, _ = cpu_alba_sample(g, roots, budget_s=30.0, max_roots=16)
+    dense = []
+    for r in roots[:4]:
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.bfs_dense_ref(g.row_ptr, g.col, int(r))
+            runs.append((time.perf_counter() - t0) * 1e3)
+        dense.append(float(np.median(runs)))
+    res["config0_rmat16_16roots"] = {
+        "nnz": g.nnz, "roots_timed": k, **var,
+        "dense_run_bfs_ref": {"hm_gteps": round(A.harmonic_mean([A.gteps(g.nnz, x) for x in dense]), 4),
+                              "median_ms_per_root": [round(x, 2) for x in dense], "threads": 1,
+                              "roots_timed": len(dense)}}
+    log(f"config 0 done ({time.time() - t:.0f}s): {res['config0_rmat16_16roots']}")
+    # config 1: RMAT s22, ef 16, 64 roots (root seed 3): the first 8
+    t = time.time()
+    g = A.Graph.rmat(22, 16, 0.57, 0.19, 0.19, seed=1, perm_seed=2)
+    roots = g.pick_roots(64, 3)
+    var, k, _ = cpu_alba_sample(g, roots, budget_s=30.0, max_roots=8)
+    res["config1_rmat22_64roots"] = {"nnz": g.nnz, "roots_timed": k, **var}
+    log(f"config 1 done ({time.time() - t:.0f}s): {res['config1_rmat22_64roots']}")
+    del g
+    # config 2: the 4096^2 grid has thousands of levels, each a scan of the
+    # layout in the restatement: the first `cap` levels of 2 roots are timed
+    # and the per-level time scaled to the root's level count
+    t = time.time()
+    g = A.Graph.grid(**W.GRID4096)
+    roots = g.pick_roots(*W.GRID4096_ROOTS, largest_cc=True)
+    s = O.build_sell(g.row_ptr, g.col, 256, 16384)
+    cap = 30
+    grid = {}
+    try:
+        levels = [int(O.bfs_queue(g.row_ptr, g.col, int(r))[0].max()) + 1 for r in roots[:2]]
+        for name, tasks in (("tasks_8", 8), ("tasks_threads", threads)):
+            per_level, est = [], []
+            for r, nl in zip(roots[:2], levels):
+                runs = []
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    got = O.albba_bfs_on(s, g.n, int(r), tasks=tasks, capacity=cap + 1, max_levels=cap)
+                    runs.append((time.perf_counter() - t0) * 1e3 / got["num_levels"])
+                per_level.append(float(np.median(runs)))
+                est.append(per_level[-1] * nl)
+            grid[name] = {"hm_gteps": round(A.harmonic_mean([A.gteps(g.nnz, x) for x in est]), 6),
+                          "ms_per_level": [round(x, 3) for x in per_level],
+                          "est_ms_per_root": [round(x, 1) for x in est]}
+    finally:
+        O.free_sell(s)
+    res["config2_grid4096_32roots"] = {"nnz": g.nnz, "roots_timed": 2, "levels": levels,
+                                       "sample": f"first {cap} levels of 2 roots, per-level median of 3 "
+                                                 "scaled to the root's level count", **grid}
+    log(f"config 2 done ({time.time() - t:.0f}s): {res['config2_grid4096_32roots']}")
+    del g
+    # config 3: the batch mix run instance after instance: 6 RMAT and 6 ER
+    # instances in full, 3 grids level-capped as above
+    t = time.time()
+    lat, nnz_s = [], []
+    n_done = {"rmat": 0, "er": 0, "grid": 0}
+    for i, (kind, p) in enumerate(W.batch_specs(256)):
+        if n_done[kind] >= (3 if kind == "grid" else 6):
+            continue
+        g = W.make_graph(kind, p)
+        r = W.root_of(kind, g, i)
+        s = O.build_sell(g.row_ptr, g.col, 256, 16384)
+        try:
+            if kind == "grid":
+                nl = int(O.bfs_queue(g.row_ptr, g.col, r)[0].max()) + 1
+                t0 = time.perf_counter()
+                got = O.albba_bfs_on(s, g.n, r, tasks=threads, capacity=cap + 1, max_levels=cap)
+                ms = (time.perf_counter() - t0) * 1e3 / got["num_levels"] * nl
+            else:
+                t0 = time.perf_counter()
+                O.albba_bfs_on(s, g.n, r, tasks=threads)
+                ms = (time.perf_counter() - t0) * 1e3
+        finally:
+            O.free_sell(s)
+        lat.append(ms)
+        nnz_s.append(g.nnz)
+        n_done[kind] += 1
+        if sum(n_done.values()) == 15:
+            break
+    res["config3_batch256"] = {"instances_timed": len(lat), "by_kind": n_done,
+                               "total_gteps_sample": round(float(np.sum(nnz_s)) / (float(np.sum(lat)) * 1e6), 4),
+                               "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+                               "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+                               "sample": f"{len(lat)} instances one after the other ({threads} tasks): "
+                                         "6 RMAT and 6 ER in full, 3 grids level-capped and scaled"}
+    log(f"config 3 done ({time.time() - t:.0f}s): {res['config3_batch256']}")
+    line = {"metric": "CPU baseline of configs 0-3 (oracle's ALBBA restatement, C + OpenMP)",
+            "value": res["config1_rmat22_64roots"]["tasks_threads"]["hm_gteps"], "unit": "GTEPS",
+            "n_gpus": 0, "steps": 1, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic, seeded",
+            "config": {"workload": "cpu_baseline_configs0-3", "threads": threads, "cpu_model": cpu_model()},
+            "cpu": res}
+    print(json.dumps(line), flush=True)
+
+
+def run_grid_workload(args):
+    """High-diameter case: 4-neighbour grid of 4096 x 4096 vertices with 10%
+    of the edges removed (seed 4, ids permuted with seed 2), 32 roots from
+    the largest component (seed 5)."""
+    import albba_amd as A
+
+    t = time.time()
+    g = A.Graph.grid(4096, 4096, 0.1, seed=4, perm_seed=2)
+    roots = g.pick_roots(args.roots if args.roots != 64 else 32, 5, largest_cc=True)
+    ctx = A.Context(0)
+    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+    log(f"grid 4096^2: n={g.n} nnz={g.nnz} ({time.time() - t:.1f}s)")
+    res = {}
+    for name, cap in (("small_frontier_kernel", None), ("per_level_launches", 0)):
+        dg.bfs(int(roots[0]), small_frontier_edges=cap, results=False)  # warm-up
+        ms, levels = [], []
+        for r in roots:
+            o = dg.bfs(int(r), small_frontier_edges=cap, results=False, capacity=1 << 15)
+            ms.append(o["bfs_ms"])
+            levels.append(o["num_levels"])
+        res[name] = {"hm_gteps": A.harmonic_mean([A.gteps(g.nnz, x) for x in ms]),
+                     "mean_ms": float(np.mean(ms)), "levels_mean": float(np.mean(levels)),
+                     "us_per_level": float(np.sum(ms) * 1e3 / np.sum(levels))}
+        log(f"  {name}: {res[name]}")
+        if args.steps <= 1:
+            break
+    R = res["small_frontier_kernel"]
+    line = {
+        "metric": "harmonic-mean GTEPS, 32 roots, 4096^2 grid (10% edges removed), heuristic switch",
+        "value": round(R["hm_gteps"], 4), "unit": "GTEPS", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(R["mean_ms"] * len(roots), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded grid, permuted ids",
+        "config": {"workload": "grid4096_p0.1_32roots", "n": g.n, "nnz_stored": g.nnz,
+                   "roots": len(roots), "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
+        "detail": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()},
+    }
+    print(json.dumps(line), flush=True)
+    dg.close()
+    ctx.close()
+
+
+def run_load_workload(args):
+    """.cbd file -> device graph ready for BFS (SELL C=64, sigma=4096,
+    pad=1): the host path (read_Dmatrix + the W = A^T transpose + the host
+    SELL build + upload) against albba_graph_load (file streamed into device
+    memory, transpose and build on the device).  The file is written first,
+    so both read it from the page cache.  RMAT of --scale (default 22 here)."""
+    import shutil
+    import tempfile
+
+    import albba_amd as A
+
+    scale = args.scale if args.scale != 24 else 22
+    g = A.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, perm_seed=2)
+    d = tempfile.mkdtemp(prefix="albba_load_")
+    path = os.path.join(d, f"rmat{scale}.cbd")
+    try:
+        g.write_cbd(path)
+        size = os.path.getsize(path)
+        log(f"wrote {path}: {size / 2**30:.2f} GiB")
+        ctx = A.Context(0)
+        runs = {"host": [], "device": []}
+        same = None
+        for rep_i in range(max(args.steps, 1) + 1):  # the first pair is a warm-up
+            t = time.perf_counter()
+            h = A.Graph.read(path)
+            t_read = time.perf_counter()
+            sell = h.sellcs(64, 4096, 1)
+            t_build = time.perf_counter()
+            dh = A.DeviceGraph(ctx, sell)
+            t_up = time.perf_counter()
+            host = {"read_ms": (t_read - t) * 1e3, "build_ms": (t_build - t_read) * 1e3,
+                    "upload_ms": (t_up - t_build) * 1e3, "total_ms": (t_up - t) * 1e3}
+            t = time.perf_counter()
+            dd = A.DeviceGraph.load(ctx, path, 64, 4096, 1)
+            dev = dict(dd.build_report, wall_ms=(time.perf_counter() - t) * 1e3)
+            if same is None:
+                ref, got = sell.arrays(), dd.layout().arrays()
+                same = all((v is None and got[k] is None) or
+                           (v is not None and np.array_equal(v, got[k])) for k, v in ref.items())
+                del ref, got
+            del sell, h
+            dh.close()
+            dd.close()
+            log(f"  host {host['total_ms']:.0f} ms, device {dev['wall_ms']:.0f} ms ({dev})")
+            if rep_i > 0:
+                runs["host"].append(host)
+                runs["device"].append(dev)
+        mean = {k: {kk: round(float(np.mean([r[kk] for r in v])), 2) for kk in v[0]}
+                for k, v in runs.items()}
+        line = {
+            "metric": f"ms from a .cbd file to a device graph ready for BFS, RMAT s{scale}",
+            "value": mean["device"]["wall_ms"], "unit": "ms", "n_gpus": 1, "steps": len(runs["host"]),
+            "warmup": 1, "ms_per_step": mean["device"]["wall_ms"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic: seeded RMAT written as .cbd (page cache)",
+            "config": {"workload": f"cbd_load_rmat_s{scale}", "n": g.n, "nnz_stored": g.nnz,
+                       "file_bytes": size, "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
+            "host_path_ms": mean["host"]["total_ms"], "detail": mean, "identical": bool(same),
+        }
+        print(json.dumps(line), flush=True)
+        ctx.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["rmat24", "grid", "batch", "load", "cpu"], default="rmat24",
+                    help="rmat24: the headline metric; grid: the 4096^2 high-diameter case; "
+                         "batch: the batched multi-instance BFS; load: .cbd file to device graph; "
+                         "cpu: the CPU baseline of configs 0-3")
+    ap.add_argument("--instances", type=int, default=256, help="batch workload size")
+    ap.add_argument("--policy", choices=["both", "heuristic", "learned"], default="both",
+                    help="run one policy only (profiling); 'both' checks bit-exactness")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a single BFS would shard as a 1D partition of the SELL slices)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "batch":
+        run_batch_workload(args)
+        return
+    if args.workload == "grid":
+        run_grid_workload(args)
+        return
+    if args.workload == "load":
+        run_load_workload(args)
+        return
+    if args.workload == "cpu":
+        run_cpu_workload(args)
+        return
+
+    import albba_amd as A
+
+    ctx = A.Context(0)
+    # the graph generated on the device (the host generator's CSR, bit for
+    # bit), with a host copy for the roots, the builders' comparison and the
+    # CPU baseline
+    dg, g = A.DeviceGraph.generate(ctx, "rmat", 64, 4096, 1, scale=args.scale, edgefactor=16,
+                                   a=0.57, b=0.19, c=0.19, seed=1, perm_seed=2)
+    gen_report = dg.build_report
+    log(f"graph RMAT s{args.scale}: n={g.n} nnz={g.nnz}, generated and laid out on the device "
+        f"({gen_report}, {dg.device_bytes / 2**30:.2f} GiB on device)")
+    roots = g.pick_roots(args.roots, 7)
+    sell_build = compare_builders(A, ctx, g)
+
+    sel = None
+    if os.path.exists(SELECTOR):
+        sel = A.Selector(ctx)
+        sel.load(SELECTOR)
+    policies = [("heuristic", A.SWITCH_HEURISTIC, None)]
+    if sel is not None:
+        policies.append(("learned", A.SWITCH_LEARNED, sel))
+    if args.policy != "both":
+        policies = [p for p in policies if p[0] == args.policy]
         if not policies:
             log(f"policy {args.policy} unavailable (no selector file)")
             sys.exit(1)

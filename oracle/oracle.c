@@ -212,6 +212,14 @@ int orc_albba_bfs(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level
                   int32_t *frontier_out, int32_t *mode_out, int64_t *bys_out, int64_t *byc_out,
                   int32_t capacity)
 {
+    return orc_albba_bfs_levels(s, root, tasks, level, parent, frontier_out, mode_out, bys_out, byc_out,
+                                capacity, INT32_MAX);
+}
+
+int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                         int32_t *frontier_out, int32_t *mode_out, int64_t *bys_out, int64_t *byc_out,
+                         int32_t capacity, int32_t max_levels)
+{
     const int32_t n = s->n, C = s->C, ns = s->nslices;
     if (root < 0 || root >= n || tasks < 1) return -1;
     const int32_t T = (int32_t)(n * 0.04); /* bfs_SELLCSv2.c:282-284 */
@@ -335,7 +343,7 @@ int orc_albba_bfs(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level
         nnzx = nnzy;
         if (nnzx > T) memset(mask_y, 0, sizeof(int32_t) * (size_t)n); /* reset */
         ++iter;
-    } while (nnzx != 0);
+    } while (nnzx != 0 && iter < max_levels);
     free(pos_of); free(mask_x); free(mask_y); free(colidx_x); free(colidx_y); free(bucket);
     free(visited); free(smax); free(smin); free(tp); free(tcnt); free(tbs); free(tbc);
     return iter;

@@ -66,6 +66,12 @@ int orc_albba_bfs(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level
                   int32_t *frontier, int32_t *mode, int64_t *bypass_slice, int64_t *bypass_col,
                   int32_t capacity);
 
+/* orc_albba_bfs stopped after at most max_levels levels (a bounded sample
+ * of a long BFS, for timing); returns the levels run. */
+int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                         int32_t *frontier, int32_t *mode, int64_t *bypass_slice, int64_t *bypass_col,
+                         int32_t capacity, int32_t max_levels);
+
 /* Graph500-style BFS tree check on W: 0 if valid, else a negative code
  *  -1 root wrong, -2 parent of a reached vertex unreached or level gap != 1,
  *  -3 tree edge missing from the graph, -4 an edge skips a level,

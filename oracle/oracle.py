@@ -53,6 +53,9 @@ def orc() -> C.CDLL:
         L.orc_albba_bfs.argtypes = [C.POINTER(OrcSell), C.c_int32, C.c_int32, I32P, I32P, I32P, I32P,
                                     I64P, I64P, C.c_int32]
         L.orc_albba_bfs.restype = C.c_int
+        L.orc_albba_bfs_levels.argtypes = [C.POINTER(OrcSell), C.c_int32, C.c_int32, I32P, I32P, I32P,
+                                           I32P, I64P, I64P, C.c_int32, C.c_int32]
+        L.orc_albba_bfs_levels.restype = C.c_int
         L.orc_validate.argtypes = [C.c_int32, I64P, I32P, C.c_int32, I32P, I32P]
         L.orc_validate.restype = C.c_int
         _orc = L
@@ -158,16 +161,19 @@ def albba_bfs(row_ptr, col, root: int, C_: int = 256, sigma: int = 16384, tasks:
         orc().orc_sell_free(C.byref(s))
 
 
-def albba_bfs_on(s: OrcSell, n: int, root: int, tasks: int = 8, capacity: int = 1 << 16):
+def albba_bfs_on(s: OrcSell, n: int, root: int, tasks: int = 8, capacity: int = 1 << 16,
+                 max_levels: int = 2 ** 31 - 1):
+    """max_levels: stop after that many levels (a bounded timing sample)."""
     lv = np.empty(n, np.int32)
     pa = np.empty(n, np.int32)
     fr = np.zeros(capacity, np.int32)
     md = np.zeros(capacity, np.int32)
     bs = np.zeros(capacity, np.int64)
     bc = np.zeros(capacity, np.int64)
-    k = orc().orc_albba_bfs(C.byref(s), root, tasks, lv.ctypes.data_as(I32P), pa.ctypes.data_as(I32P),
-                            fr.ctypes.data_as(I32P), md.ctypes.data_as(I32P), bs.ctypes.data_as(I64P),
-                            bc.ctypes.data_as(I64P), capacity)
+    k = orc().orc_albba_bfs_levels(C.byref(s), root, tasks, lv.ctypes.data_as(I32P),
+                                   pa.ctypes.data_as(I32P), fr.ctypes.data_as(I32P),
+                                   md.ctypes.data_as(I32P), bs.ctypes.data_as(I64P),
+                                   bc.ctypes.data_as(I64P), capacity, max_levels)
     if k < 0:
         raise RuntimeError(f"orc_albba_bfs failed: {k}")
     m = min(k, capacity)
