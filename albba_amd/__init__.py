@@ -96,7 +96,8 @@ class _Stats(C.Structure):
     _fields_ = [("num_levels", C.c_int32), ("capacity", C.c_int32), ("reached", C.c_int64),
                 ("edges_in_component", C.c_int64), ("bfs_ms", C.c_double), ("frontier", I32P),
                 ("choice", I32P), ("level_ms", F32P), ("features", F32P), ("choice_ms", F32P),
-                ("entries", I64P), ("words", I64P), ("bypassed", I64P), ("kernel_ms", F32P)]
+                ("entries", I64P), ("words", I64P), ("bypassed", I64P), ("kernel_ms", F32P),
+                ("col_bypassed", I64P)]
 
 
 class _BatchResult(C.Structure):
@@ -573,10 +574,11 @@ class DeviceGraph:
         wd = np.zeros(cap, np.int64)
         by = np.zeros(cap, np.int64)
         km = np.zeros(cap, np.float32)
+        cb = np.zeros(cap, np.int64)
         st = _Stats(0, cap, 0, 0, 0.0, fr.ctypes.data_as(I32P), ch.ctypes.data_as(I32P),
                     lm.ctypes.data_as(F32P), ft.ctypes.data_as(F32P), cm.ctypes.data_as(F32P),
                     en.ctypes.data_as(I64P), wd.ctypes.data_as(I64P), by.ctypes.data_as(I64P),
-                    km.ctypes.data_as(F32P))
+                    km.ctypes.data_as(F32P), cb.ctypes.data_as(I64P))
         _check(lib().albba_bfs(self.ctx._p, self._p, int(root), C.byref(o), C.byref(st)))
         L = min(st.num_levels, cap)
         out = {
@@ -586,6 +588,7 @@ class DeviceGraph:
             "features": ft[:L * NUM_FEATURES].reshape(L, NUM_FEATURES).copy(),
             "choice_ms": cm[:L * 4].reshape(L, 4).copy(), "entries": en[:L].copy(),
             "words": wd[:L].copy(), "bypassed": by[:L].copy(), "kernel_ms": km[:L].copy(),
+            "col_bypassed": cb[:L].copy(),
         }
         if results:
             out["level"], out["parent"] = self.results()

@@ -31,7 +31,8 @@ typedef struct albd_slot {
     unsigned long long scanned;      /* pull: column entries gathered         */
     unsigned long long words;        /* pull: position words scanned          */
     unsigned long long bypassed;     /* pull: words skipped, all visited      */
-    unsigned long long pad_[2];      /* one slot per 64 bytes                 */
+    unsigned long long col_bypassed; /* pull + column bypass: gathers skipped */
+    unsigned long long pad_;         /* one slot per 64 bytes                 */
 } albd_slot;
 
 typedef struct albd_counters {
@@ -48,6 +49,8 @@ typedef struct albd_counters {
     unsigned long long scanned;        /* pull: column entries gathered        */
     unsigned long long words;          /* pull: position words scanned         */
     unsigned long long bypassed;       /* pull: words bypassed, all visited    */
+    unsigned long long col_bypassed;   /* pull + column bypass: column entries
+                                          whose frontier word was empty        */
     int sel_choice;                    /* policy's choice for the next level   */
     int heur_choice;                   /* the reference heuristic's choice     */
     float features[12];                /* selector features of the next level  */

@@ -70,7 +70,8 @@ __device__ __forceinline__ void slot_add(albd_slot *slots, unsigned long long cn
                                          unsigned long long edges, unsigned long long words,
                                          unsigned long long scanned,
                                          unsigned long long words_scanned = 0,
-                                         unsigned long long bypassed = 0)
+                                         unsigned long long bypassed = 0,
+                                         unsigned long long col_bypassed = 0)
 {
     albd_slot *s = &slots[(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) % ALBD_NSLOT];
     if (cnt) atomicAdd(&s->next_count, cnt);
@@ -78,6 +79,7 @@ __device__ __forceinline__ void slot_add(albd_slot *slots, unsigned long long cn
     if (words) atomicAdd(&s->next_words, words);
     if (scanned) atomicAdd(&s->scanned, scanned);
     if (words_scanned) atomicAdd(&s->words, words_scanned);
+    if (col_bypassed) atomicAdd(&s->col_bypassed, col_bypassed);
     if (bypassed) atomicAdd(&s->bypassed, bypassed);
 }
 
@@ -127,6 +129,9 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
 
     const bool sym = g.pull_col == g.push_col;
     unsigned long long cnt = 0, nzw = 0, scanned = 0, edges = 0, wscan = 0, wbyp = 0;
+    // column bypass: this lane's column entries whose frontier word the
+    // summary shows empty (their gathers are skipped)
+    unsigned int cbs = 0;
     // One word ahead: while a word's columns are scanned, the next word is
     // claimed, its list entry read, and then its row lengths, visited word
     // and slice pointer loaded, so that a word starts with its first column
@@ -232,8 +237,10 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             const int c = cidx[u];
                             const int wd = c >> 6;
                             bool maybe = true;
-                            if constexpr (VAR == ALBD_PULL_LIST_CB)
+                            if constexpr (VAR == ALBD_PULL_LIST_CB) {
                                 maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                                cbs += maybe ? 0u : 1u;
+                            }
                             if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
                         }
                         hit[u] = hh;
@@ -286,8 +293,10 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             c = g.pull_col[rs + e];
                             const int wd = c >> 6;
                             bool maybe = true;
-                            if constexpr (VAR == ALBD_PULL_LIST_CB)
+                            if constexpr (VAR == ALBD_PULL_LIST_CB) {
                                 maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                                cbs += maybe ? 0u : 1u;
+                            }
                             if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
                         }
                         scanned += __popcll(__ballot(live_g && e < rl));
@@ -329,7 +338,9 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
         h = h1;
     }
     edges = wave_sum(edges);
-    if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp);
+    unsigned long long cb_total = 0;
+    if constexpr (VAR == ALBD_PULL_LIST_CB) cb_total = wave_sum((unsigned long long)cbs);
+    if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp, cb_total);
 }
 
 template <int C, int VAR>
@@ -438,7 +449,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
     static_assert(kCtrWords <= 64, "one wave moves the counter block");
     __shared__ float xs[ALBBA_MLP_IN], h1[ALBBA_MLP_H1], h2[ALBBA_MLP_H2], ys[ALBBA_MLP_OUT];
     __shared__ float s_params[ALBBA_MLP_NPARAM];
-    __shared__ unsigned long long s_red[6][kSelectBlock / 64];
+    __shared__ unsigned long long s_red[7][kSelectBlock / 64];
     __shared__ union {
         albd_counters c;
         unsigned long long w[sizeof(albd_counters) / 8];
@@ -451,7 +462,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
     // (one lane per word: thread 0 alone would chain one load per word).
     constexpr int kPer = ALBD_NSLOT / kSelectBlock;
     static_assert(ALBD_NSLOT % kSelectBlock == 0, "slots split evenly over the block");
-    static_assert(sizeof(albd_slot) == 64, "slot = 3 x 16 counter bytes + padding");
+    static_assert(sizeof(albd_slot) == 64, "slot = 4 x 16 counter bytes");
     // a used unvisited-word list (slot (level - 1) % 3, built by the
     // previous level's preparation) becomes the current list
     const bool list_used = !sel.first && (sel.prev_choice == ALBBA_CHOICE_PULL_SB ||
@@ -463,23 +474,24 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
     if (tid == 0) st.ctr->list_cnt[(sel.level + 1) % 3] = 0;
     float pr[mlp_per_thread<kSelectBlock>()];
     if (sel.params) mlp_load<kSelectBlock>(pr, sel.params);
-    ulonglong2 v[kPer][3];
+    ulonglong2 v[kPer][4];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(&st.slots[tid + k * kSelectBlock]);
         v[k][0] = p[0];
         v[k][1] = p[1];
         v[k][2] = p[2];
+        v[k][3] = p[3];
     }
     // read once, by this thread only: the next level starts from zero
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         ulonglong2 *p = reinterpret_cast<ulonglong2 *>(&st.slots[tid + k * kSelectBlock]);
-        p[0] = p[1] = p[2] = make_ulonglong2(0ull, 0ull);
+        p[0] = p[1] = p[2] = p[3] = make_ulonglong2(0ull, 0ull);
     }
     if (tid < kCtrWords) s_ctr.w[tid] = cw;
     if (sel.params) mlp_store<kSelectBlock>(s_params, pr);
-    unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0;
+    unsigned long long F = 0, mf = 0, nzw = 0, scn = 0, wds = 0, byp = 0, cbp = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         F += v[k][0].x;   // next_count
@@ -488,6 +500,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
         scn += v[k][1].y; // scanned
         wds += v[k][2].x; // words
         byp += v[k][2].y; // bypassed
+        cbp += v[k][3].x; // col_bypassed
     }
     F = wave_sum(F);
     mf = wave_sum(mf);
@@ -495,6 +508,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
     scn = wave_sum(scn);
     wds = wave_sum(wds);
     byp = wave_sum(byp);
+    cbp = wave_sum(cbp);
     if (lane == 0) {
         s_red[0][wid] = F;
         s_red[1][wid] = mf;
@@ -502,12 +516,13 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
         s_red[3][wid] = scn;
         s_red[4][wid] = wds;
         s_red[5][wid] = byp;
+        s_red[6][wid] = cbp;
     }
     __syncthreads();
     if (wid != 0) return;
     albd_counters &c = s_ctr.c;
     // the totals in every lane of wave 0
-    F = mf = nzw = scn = wds = byp = 0;
+    F = mf = nzw = scn = wds = byp = cbp = 0;
     for (int i = 0; i < kSelectBlock / 64; ++i) {
         F += s_red[0][i];
         mf += s_red[1][i];
@@ -515,6 +530,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
         scn += s_red[3][i];
         wds += s_red[4][i];
         byp += s_red[5][i];
+        cbp += s_red[6][i];
     }
     const unsigned long long reached = c.reached + F;
     const unsigned long long redges = c.reached_edges + mf;
@@ -531,6 +547,7 @@ __device__ __forceinline__ void select_body(const albd_graph &g, const albd_stat
         c.scanned = scn;
         c.words = wds;
         c.bypassed = byp;
+        c.col_bypassed = cbp;
         c.reached = reached;
         c.reached_edges = redges;
         const int heur = heuristic_choice(F, g.n);

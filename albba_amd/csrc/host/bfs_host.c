@@ -737,7 +737,7 @@ typedef struct loop_state {
 } loop_state;
 
 typedef struct level_sums {
-    int64_t next_count, next_edges, next_words, scanned, words, bypassed;
+    int64_t next_count, next_edges, next_words, scanned, words, bypassed, col_bypassed;
     int64_t qnext_len, active_len, queue_len, push_edges;
 } level_sums;
 
@@ -750,6 +750,7 @@ static void sum_counters(const albd_counters *h, level_sums *o)
     o->scanned = (int64_t)h->scanned;
     o->words = (int64_t)h->words;
     o->bypassed = (int64_t)h->bypassed;
+    o->col_bypassed = (int64_t)h->col_bypassed;
     o->qnext_len = (int64_t)h->qnext_len;
     o->active_len = (int64_t)h->active_len;
     o->queue_len = (int64_t)h->queue_len;
@@ -1189,7 +1190,10 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                     if (stats->entries) stats->entries[lv] = push ? sums.push_edges : sums.scanned;
                     if (stats->words) stats->words[lv] = push ? sums.queue_len : sums.words;
                 }
-                if (stats->bypassed) stats->bypassed[lv] = push ? 0 : sums.bypassed;
+                /* words the pull did not scan: off the unvisited-word list,
+                 * or on it but visited or empty by now */
+                if (stats->bypassed) stats->bypassed[lv] = push || small ? 0 : (int64_t)G->g.nwords - sums.words;
+                if (stats->col_bypassed) stats->col_bypassed[lv] = push || small ? 0 : sums.col_bypassed;
             }
         }
         if (!spec) commit_pointers(G, &ls, choice, 1);

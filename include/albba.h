@@ -241,10 +241,18 @@ typedef struct albba_bfs_stats {
                                     push: frontier edges expanded (or NULL)       */
     int64_t *words;              /* [capacity] pull: position words scanned;
                                     push: queue length (or NULL)                  */
-    int64_t *bypassed;           /* [capacity] pull: words skipped because every
-                                    lane is visited (slice bypass) (or NULL)      */
+    int64_t *bypassed;           /* [capacity] pull: 64-row words not scanned
+                                    because every row in them is visited or
+                                    empty (the reference's slice bypass,
+                                    bfs/src/bfs_par_kernels.c:152-153); push: 0
+                                    (or NULL)                                     */
     float *kernel_ms;            /* [capacity] main kernel only (pull or push
                                     expansion), needs time_levels (or NULL)       */
+    int64_t *col_bypassed;       /* [capacity] pull with column bypass: column
+                                    entries whose frontier word was empty, so
+                                    their gather was skipped (the reference's
+                                    width bypass, bfs_par_kernels.c:613,628-631,
+                                    counts skipped columns); otherwise 0 (or NULL) */
 } albba_bfs_stats;
 
 /* Last error message of the calling thread (static storage). */

@@ -266,3 +266,28 @@ def test_small_frontier_push_runs(ctx, cap, blocks):
             assert (o["choice"][o["frontier"] > 0] >= 0).all()
             check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PUSH, small_frontier_edges=cap)
         dg.close()
+
+
+def test_bypass_counters(ctx):
+    """Per-level bypass counts (reported, as the reference's bypass_slice and
+    bypass_width): a pull skips the 64-row words that are off the unvisited
+    list or fully visited, the column-bypass pull also skips the gathers into
+    empty frontier words, and the plain pulls skip none; levels and parents
+    stay those of the oracle."""
+    g = A.Graph.rmat(15, 16, seed=121, perm_seed=122)
+    rp, col = g.row_ptr.copy(), g.col.copy()
+    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+    nwords = -(-g.n // 64)
+    root = int(g.pick_roots(1, 123)[0])
+    cb = check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PULL_CB)
+    sb = check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PULL_SB)
+    for o in (cb, sb):
+        np.testing.assert_array_equal(o["bypassed"], nwords - o["words"])
+        assert (o["bypassed"] >= 0).all() and o["bypassed"][-1] > 0
+    assert (sb["col_bypassed"] == 0).all()
+    assert (cb["col_bypassed"] <= cb["entries"]).all() and cb["col_bypassed"].sum() > 0
+    # the same rows are scanned either way: a skipped gather never hits
+    np.testing.assert_array_equal(cb["entries"], sb["entries"])
+    push = check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PUSH)
+    assert (push["bypassed"] == 0).all() and (push["col_bypassed"] == 0).all()
+    dg.close()
