@@ -202,6 +202,8 @@ def lib() -> C.CDLL:
         "albba_selector_set_params": (C.c_int, [C.c_void_p, F32P]),
         "albba_selector_load": (C.c_int, [C.c_void_p, C.c_char_p]),
         "albba_selector_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "albba_selector_set_queue_ratio": (C.c_int, [C.c_void_p, C.c_float]),
+        "albba_selector_queue_ratio": (C.c_float, [C.c_void_p]),
         "albba_selector_train": (C.c_int, [C.c_void_p, F32P, F32P, C.c_int32, C.c_int32, C.c_float,
                                            F32P]),
         "albba_selector_loss_grad": (C.c_int, [C.c_void_p, F32P, F32P, C.c_int32, F32P, F32P]),
@@ -719,6 +721,16 @@ class Selector:
 
     def load(self, path: str):
         _check(lib().albba_selector_load(self._p, path.encode()))
+
+    @property
+    def queue_ratio(self) -> float:
+        """Push levels list the next queue while their frontier out-edges
+        are at most queue_ratio * n (0: the heuristic's threshold)."""
+        return float(lib().albba_selector_queue_ratio(self._p))
+
+    @queue_ratio.setter
+    def queue_ratio(self, r: float):
+        _check(lib().albba_selector_set_queue_ratio(self._p, float(r)))
 
     def save(self, path: str):
         _check(lib().albba_selector_save(self._p, path.encode()))

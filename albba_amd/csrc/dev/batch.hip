@@ -285,67 +285,79 @@ __device__ bool run_instance(BLds &S, const albd_batch_item &it, int rank, int T
             }
         } else {
             ++pulls;
-            // pull: workgroup `rank` takes words [w0, w1); wave wid takes every
-            // kBWaves-th word; unvisited lanes look for their row's first
+            // pull: workgroup `rank` takes words [w0, w1) in runs of 64; wave
+            // wid takes every kBWaves-th run.  The wave's lanes test the 64
+            // words of a run at once (rows with a length, not visited: every
+            // vertex found so far has its visited bit, set by its
+            // discoverer), then the wave works through the words that have
+            // such rows; their unvisited lanes look for their row's first
             // neighbour in B_L (rows list neighbours by ascending id)
             const int64_t per = (nwords + T - 1) / T;
             const int64_t w0 = (int64_t)rank * per, w1 = w0 + per < nwords ? w0 + per : nwords;
-            for (int64_t w = w0 + wid; w < w1; w += kBWaves) {
-                const uint64_t hl = g.haslen[w];
-                if (hl == 0) continue;
-                // every vertex found so far has its visited bit (set by its discoverer)
-                const uint64_t unres = hl & ~ld_coherent(&it.visited[w]);
-                if (unres == 0) continue;
-                const int64_t p = w * 64 + lane;
-                int32_t par = -1;
-                if (it.mapping == ALBD_MAP_LANE_ROW) {
-                    if ((unres >> lane) & 1ull) {
-                        const int64_t r1 = g.pull_ptr[p + 1];
-                        for (int64_t k = g.pull_ptr[p]; k < r1; ++k) {
-                            const int32_t u = g.pull_col[k];
-                            if ((ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull) {
-                                par = g.row_order[u];
-                                break;
-                            }
-                        }
-                    }
-                } else {
-                    // the wave scans one row at a time, 64 entries per step
-                    uint64_t todo = unres;
-                    while (todo) {
-                        const int r = __ffsll((unsigned long long)todo) - 1;
-                        todo &= todo - 1;
-                        const int64_t rs = g.pull_ptr[w * 64 + r], re = g.pull_ptr[w * 64 + r + 1];
-                        for (int64_t k = rs; k < re; k += 64) {
-                            const int64_t kk = k + lane;
-                            bool h = false;
-                            int32_t u = 0;
-                            if (kk < re) {
-                                u = g.pull_col[kk];
-                                h = (ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull;
-                            }
-                            const uint64_t hm = __ballot(h);
-                            if (hm) {
-                                const int32_t first = __shfl(u, __ffsll((unsigned long long)hm) - 1, 64);
-                                if (lane == r) par = g.row_order[first];
-                                break;
-                            }
-                        }
-                    }
+            for (int64_t run = w0 + (int64_t)wid * 64; run < w1; run += (int64_t)kBWaves * 64) {
+                const int64_t wl = run + lane;
+                uint64_t unres_l = 0;
+                if (wl < w1) {
+                    const uint64_t hl = g.haslen[wl];
+                    if (hl) unres_l = hl & ~ld_coherent(&it.visited[wl]);
                 }
-                bool won = false;
-                if (par >= 0) won = atomicMin(&it.key[p], lkey | (uint32_t)par) == ~0ull;
-                const uint64_t fm = __ballot(won);
-                if (fm) {
-                    if (lane == 0) {
-                        const unsigned long long old =
-                            atomicOr(reinterpret_cast<unsigned long long *>(&bnext[w]), fm);
-                        atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[w]), fm);
-                        my_w += old == 0 ? 1 : 0;
+                uint64_t todo_w = __ballot(unres_l != 0);
+                while (todo_w) {
+                    const int kw = __ffsll((unsigned long long)todo_w) - 1;
+                    todo_w &= todo_w - 1;
+                    const int64_t w = run + kw;
+                    const uint64_t unres = __shfl(unres_l, kw, 64);
+                    const int64_t p = w * 64 + lane;
+                    int32_t par = -1;
+                    if (it.mapping == ALBD_MAP_LANE_ROW) {
+                        if ((unres >> lane) & 1ull) {
+                            const int64_t r1 = g.pull_ptr[p + 1];
+                            for (int64_t k = g.pull_ptr[p]; k < r1; ++k) {
+                                const int32_t u = g.pull_col[k];
+                                if ((ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull) {
+                                    par = g.row_order[u];
+                                    break;
+                                }
+                            }
+                        }
+                    } else {
+                        // the wave scans one row at a time, 64 entries per step
+                        uint64_t todo = unres;
+                        while (todo) {
+                            const int r = __ffsll((unsigned long long)todo) - 1;
+                            todo &= todo - 1;
+                            const int64_t rs = g.pull_ptr[w * 64 + r], re = g.pull_ptr[w * 64 + r + 1];
+                            for (int64_t k = rs; k < re; k += 64) {
+                                const int64_t kk = k + lane;
+                                bool h = false;
+                                int32_t u = 0;
+                                if (kk < re) {
+                                    u = g.pull_col[kk];
+                                    h = (ld_coherent(&bcur[u >> 6]) >> (u & 63)) & 1ull;
+                                }
+                                const uint64_t hm = __ballot(h);
+                                if (hm) {
+                                    const int32_t first = __shfl(u, __ffsll((unsigned long long)hm) - 1, 64);
+                                    if (lane == r) par = g.row_order[first];
+                                    break;
+                                }
+                            }
+                        }
                     }
-                    if (won) {
-                        my_f += 1;
-                        my_e += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
+                    bool won = false;
+                    if (par >= 0) won = atomicMin(&it.key[p], lkey | (uint32_t)par) == ~0ull;
+                    const uint64_t fm = __ballot(won);
+                    if (fm) {
+                        if (lane == 0) {
+                            const unsigned long long old =
+                                atomicOr(reinterpret_cast<unsigned long long *>(&bnext[w]), fm);
+                            atomicOr(reinterpret_cast<unsigned long long *>(&it.visited[w]), fm);
+                            my_w += old == 0 ? 1 : 0;
+                        }
+                        if (won) {
+                            my_f += 1;
+                            my_e += (unsigned long long)(g.push_ptr[p + 1] - g.push_ptr[p]);
+                        }
                     }
                 }
             }

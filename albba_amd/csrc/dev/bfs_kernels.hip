@@ -27,6 +27,10 @@
 
 namespace {
 
+#ifndef ALBD_PULL_PIPE
+#define ALBD_PULL_PIPE 0
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 
@@ -196,6 +200,97 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                 const uint64_t start = unres;
                 int parent_pos = -1;
                 int j = 0, step = 1;
+#if ALBD_PULL_PIPE
+                // columns are loaded 1, 2, then 4 at a time: most lanes of a
+                // dense level hit in the first column, the survivors tend to
+                // run long.  Each step's columns are loaded right behind the
+                // previous step's frontier gathers, for the lanes still
+                // unresolved before that step, so that the column loads of
+                // one step overlap the gathers of the step before.
+                int cidx[4];
+                bool av[4];
+                auto load_cols = [&](int j0, int st, int (&ci)[4], bool (&a_)[4]) {
+                    const bool mn = (unres >> lane) & 1ull;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        a_[u] = false;
+                        ci[u] = 0;
+                        if (u < st) {
+                            const int col = j0 + u;
+                            int act;
+                            if constexpr (C <= 64)
+                                act = __popcll(__ballot(col < len) & gmask);
+                            else
+                                act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
+                            a_[u] = mn && col < len;
+                            // streaming: non-temporal, so that the frontier
+                            // and visited bitmaps stay in L2
+                            if (a_[u]) ci[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
+                            cp += (act + padm) & ~padm;
+                        }
+                    }
+                };
+                bool go = __popcll(__ballot(((unres >> lane) & 1ull) && j < len)) > g.sparse_lanes;
+                if (go) load_cols(0, 1, cidx, av);
+                while (go) {
+                    bool hit[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        bool hh = false;
+                        if (av[u]) {
+                            const int c = cidx[u];
+                            const int wd = c >> 6;
+                            bool maybe = true;
+                            if constexpr (VAR == ALBD_PULL_LIST_CB) {
+                                maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
+                                cbs += maybe ? 0u : 1u;
+                            }
+                            if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
+                        }
+                        hit[u] = hh;
+                    }
+                    // the next step's columns, behind this step's gathers
+                    const int nstep = step < 4 ? 2 * step : 4;
+                    int ncidx[4];
+                    bool nav[4];
+                    const bool more = __ballot(((unres >> lane) & 1ull) && j + step < len) != 0;
+                    if (more) {
+                        load_cols(j + step, nstep, ncidx, nav);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            nav[u] = false;
+                            ncidx[u] = 0;
+                        }
+                    }
+                    // the next word's head travels with this word's first
+                    // columns
+                    if (!ahead) {
+                        h1 = load_head(w1);
+                        ahead = true;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (u < step) {
+                            const bool hh = hit[u] && ((unres >> lane) & 1ull);
+                            if (hh) parent_pos = cidx[u];
+                            unres &= ~__ballot(hh);
+                            scanned += __popcll(__ballot(av[u]));
+                        }
+                    }
+                    j += step;
+                    step = nstep;
+                    // few live lanes left: their rows go to the row phase below
+                    go = more && unres != 0 &&
+                         __popcll(__ballot(((unres >> lane) & 1ull) && j < len)) > g.sparse_lanes;
+                    // columns loaded for lanes this step resolved are dropped
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        cidx[u] = ncidx[u];
+                        av[u] = nav[u] && ((unres >> lane) & 1ull);
+                    }
+                }
+#else
                 // columns are loaded 1, 2, then 4 at a time: most lanes of a
                 // dense level hit in the first column, the survivors tend to
                 // run long
@@ -258,6 +353,7 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                     if (unres == 0) break;
                     step = step < 4 ? 2 * step : 4;
                 }
+#endif
                 if (!ahead) {
                     h1 = load_head(w1);
                     ahead = true;
@@ -343,8 +439,15 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp, cb_total);
 }
 
+#if ALBD_PULL_PIPE
+// 8 waves per SIMD: two 1024-thread blocks per CU for the column-bypass
+// pull (64 VGPRs at most)
+#define ALBD_PULL_WAVES __attribute__((amdgpu_waves_per_eu(8)))
+#else
+#define ALBD_PULL_WAVES
+#endif
 template <int C, int VAR>
-__global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
+__global__ __launch_bounds__(pull_block<VAR>()) ALBD_PULL_WAVES void k_pull(PullArgs a)
 {
     extern __shared__ uint32_t lds_sum[];
     if (!spec_go(a.st.ctr, Spec{a.spec, a.spec_choice, ~0ull})) return;

@@ -978,8 +978,10 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * the next level is likely a pull, which needs no queue (a push after it
      * lists the bitmap instead).  The heuristic pushes exactly while the
      * frontier is at most push_vertex_cap; the learned selector turns to
-     * pulls far earlier (on RMAT s24 from about 2 000 frontier vertices). */
-    const int64_t learned_qmax = G->n / 8192 > 1024 ? G->n / 8192 : 1024;
+     * pulls earlier, at the ratio its training chose (see
+     * albba_selector_set_queue_ratio). */
+    const float qratio = policy == ALBBA_SWITCH_LEARNED ? albba_selector_queue_ratio(opts.selector) : 0.0f;
+    const int64_t learned_qmax = qratio > 0.0f ? (int64_t)ceil((double)qratio * (double)G->n) : push_vertex_cap;
     const int64_t queue_max_policy = policy == ALBBA_SWITCH_LEARNED ? learned_qmax : push_vertex_cap;
     /* a scripted level lists the queue exactly when the script pushes next
      * (-1: never; past the script the heuristic's rule) */

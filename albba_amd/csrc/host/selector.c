@@ -6,8 +6,11 @@
  * and Adam run on the GPU (mlp.hip); this file owns the parameters, the
  * feature vector, the training loop and the weight file format.
  *
- * Weight file: a first line "albba-selector 1 1604" then one parameter per
- * line printed with %.9g (exact float32 round trip).
+ * Weight file: a first line "albba-selector 2 1604", one parameter per line
+ * printed with %.9g (exact float32 round trip), then "queue_ratio <r>" (the
+ * push levels' queue-listing bound, frontier out-edges per vertex; see
+ * albba_selector_set_queue_ratio).  Version 1 files end after the parameters
+ * and leave the ratio unset.
  */
 #include "internal.h"
 #include "selector_int.h"
@@ -18,6 +21,7 @@ struct albba_selector {
     albba_ctx *ctx;
     float *dparams; /* device copy */
     float params[ALBBA_MLP_NPARAM];
+    float queue_ratio; /* 0: unset */
 };
 
 static int upload(albba_selector *s)
@@ -78,8 +82,9 @@ int albba_selector_save(const albba_selector *s, const char *path)
     if (!s || !path) return albba_fail(ALBBA_EINVAL, "selector_save: null");
     FILE *f = fopen(path, "w");
     if (!f) return albba_fail(ALBBA_EIO, "selector_save: cannot open %s", path);
-    fprintf(f, "albba-selector 1 %d\n", ALBBA_MLP_NPARAM);
+    fprintf(f, "albba-selector 2 %d\n", ALBBA_MLP_NPARAM);
     for (int i = 0; i < ALBBA_MLP_NPARAM; ++i) fprintf(f, "%.9g\n", (double)s->params[i]);
+    fprintf(f, "queue_ratio %.9g\n", (double)s->queue_ratio);
     int bad = ferror(f);
     fclose(f);
     return bad ? albba_fail(ALBBA_EIO, "selector_save: write error") : ALBBA_OK;
@@ -94,7 +99,7 @@ int albba_selector_load(albba_selector *s, const char *path)
     int ver = 0, np = 0;
     float tmp[ALBBA_MLP_NPARAM];
     if (fscanf(f, "%31s %d %d", magic, &ver, &np) != 3 || strcmp(magic, "albba-selector") ||
-        ver != 1 || np != ALBBA_MLP_NPARAM) {
+        (ver != 1 && ver != 2) || np != ALBBA_MLP_NPARAM) {
         fclose(f);
         return albba_fail(ALBBA_EIO, "selector_load: %s is not a selector file", path);
     }
@@ -104,10 +109,29 @@ int albba_selector_load(albba_selector *s, const char *path)
             return albba_fail(ALBBA_EIO, "selector_load: bad parameter %d", i);
         }
     }
+    float ratio = 0.0f;
+    if (ver == 2) {
+        char key[32];
+        if (fscanf(f, "%31s %f", key, &ratio) != 2 || strcmp(key, "queue_ratio") || !isfinite(ratio) ||
+            ratio < 0.0f) {
+            fclose(f);
+            return albba_fail(ALBBA_EIO, "selector_load: bad queue_ratio line in %s", path);
+        }
+    }
     fclose(f);
     memcpy(s->params, tmp, sizeof tmp);
+    s->queue_ratio = ratio;
     return upload(s);
 }
+
+int albba_selector_set_queue_ratio(albba_selector *s, float ratio)
+{
+    if (!s || !isfinite(ratio) || ratio < 0.0f) return albba_fail(ALBBA_EINVAL, "selector_set_queue_ratio: bad argument");
+    s->queue_ratio = ratio;
+    return ALBBA_OK;
+}
+
+float albba_selector_queue_ratio(const albba_selector *s) { return s ? s->queue_ratio : 0.0f; }
 
 /* device buffers for a batch of samples */
 typedef struct batch_buf {

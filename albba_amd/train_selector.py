@@ -6,13 +6,24 @@ all-choices policy (ALBBA_SWITCH_BEST) is executed once per kernel choice on
 a snapshot of the state -- push SpMSpV, pull SpMV, pull over the unvisited
 slice list, pull with frontier-word column bypass -- and the median kernel
 time of each choice is recorded next to the 12 level features.  Graphs are
-RMAT of scale 18-22 with a in [0.45, 0.65] (b = c = (1 - a - 0.05) / 2,
-d = 0.05) and edge factor 8-32, drawn from train_seed.
+drawn from train_seed: RMAT of scale 18-22 with a in [0.45, 0.65]
+(b = c = (1 - a - 0.05) / 2, d = 0.05) and edge factor 8-32, Erdos-Renyi
+graphs of 2^18-2^20 vertices with mean degree 4-16, and square grids of
+side 256-1024 with 10% of the edges removed.
 
 Targets are relative log times, t'_c = log t_c - mean_k log t_k (each time
 capped at 1000x the level's fastest), so the network ranks the choices; the
 BFS runs the argmin.  The MLP 12 -> 32 -> 32 -> 4 is trained full-batch with
 Adam by the library's own HIP forward/backward kernels.
+
+Model selection: several networks (different initial weights) are trained
+on the training graphs' levels; each runs the deployed level loop on
+validation graphs that no benchmark or test uses (RMAT s23 and s24 and an
+Erdos-Renyi graph with seeds of their own), and the one with the highest
+harmonic-mean GTEPS over those graphs is kept.  Its queue-listing ratio
+(see albba_selector_set_queue_ratio) is the largest frontier out-edges per
+vertex at which, on the validation graphs, a push level was followed by
+another push; it is stored with the weights.
 
     python -m albba_amd.train_selector --out albba_amd/selector.txt
 """
@@ -27,9 +38,19 @@ import numpy as np
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import albba_amd as A  # noqa: E402
 
+# validation graphs: seeds used by no benchmark, test or training graph
+VALIDATION = [("rmat", dict(scale=23, edgefactor=16, a=0.57, b=0.19, c=0.19, seed=101, perm_seed=102)),
+              ("rmat", dict(scale=24, edgefactor=16, a=0.57, b=0.19, c=0.19, seed=201, perm_seed=202)),
+              ("rmat", dict(scale=22, edgefactor=32, a=0.5, b=0.225, c=0.225, seed=301, perm_seed=302)),
+              ("er", dict(n=1 << 22, mean_degree=16.0, seed=401, perm_seed=402))]
 
-def graph_specs(train_seed: int, scales, per_scale: int):
-    """Deterministic (scale, edgefactor, a, graph_seed, perm_seed) list."""
+
+def u01(x):
+    return (x >> 11) / float(1 << 53)
+
+
+def graph_specs(train_seed: int, scales, per_scale: int, er: int, grids: int):
+    """Deterministic list of (kind, generator arguments)."""
     specs = []
     k = 0
     for s in scales:
@@ -37,8 +58,21 @@ def graph_specs(train_seed: int, scales, per_scale: int):
             r = [A.rng_u64(train_seed, k * 8 + i) for i in range(4)]
             k += 1
             ef = 8 + int(r[0] % 25)                       # 8 .. 32
-            a = 0.45 + 0.20 * (r[1] >> 11) / float(1 << 53)  # 0.45 .. 0.65
-            specs.append((s, ef, round(a, 4), int(r[2] % 1000003) + 11, int(r[3] % 1000003) + 13))
+            a = round(0.45 + 0.20 * u01(r[1]), 4)         # 0.45 .. 0.65
+            b = (1.0 - a - 0.05) / 2.0
+            specs.append(("rmat", dict(scale=s, edgefactor=ef, a=a, b=b, c=b,
+                                       seed=int(r[2] % 1000003) + 11, perm_seed=int(r[3] % 1000003) + 13)))
+    for _ in range(er):
+        r = [A.rng_u64(train_seed, k * 8 + i) for i in range(4)]
+        k += 1
+        specs.append(("er", dict(n=1 << (18 + int(r[0] % 3)), mean_degree=round(4 + 12 * u01(r[1]), 3),
+                                 seed=int(r[2] % 1000003) + 11, perm_seed=int(r[3] % 1000003) + 13)))
+    for _ in range(grids):
+        r = [A.rng_u64(train_seed, k * 8 + i) for i in range(4)]
+        k += 1
+        side = 256 + int(r[0] % 769)
+        specs.append(("grid", dict(rows=side, cols=side, p_remove=0.1,
+                                   seed=int(r[2] % 1000003) + 11, perm_seed=int(r[3] % 1000003) + 13)))
     return specs
 
 
@@ -48,21 +82,19 @@ def heuristic_choice(frontier: int, n: int) -> int:
 
 def collect(ctx, specs, roots_per_graph: int, reps: int, root_seed: int, log=print):
     X, T, H, G = [], [], [], []
-    for gi, (scale, ef, a, gseed, pseed) in enumerate(specs):
-        b = c = (1.0 - a - 0.05) / 2.0
+    for gi, (kind, p) in enumerate(specs):
         t0 = time.time()
-        g = A.Graph.rmat(scale, ef, a, b, c, seed=gseed, perm_seed=pseed)
-        dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
-        for r in g.pick_roots(roots_per_graph, root_seed + gi):
-            o = dg.bfs(int(r), policy=A.SWITCH_BEST, timing_reps=reps, results=False)
+        dg, g = A.DeviceGraph.generate(ctx, kind, 64, 4096, 1, **p)
+        for r in g.pick_roots(roots_per_graph, root_seed + gi, largest_cc=(kind == "grid")):
+            o = dg.bfs(int(r), policy=A.SWITCH_BEST, timing_reps=reps, results=False,
+                       capacity=1 << 14)
             for lv in range(o["num_levels"]):
                 X.append(o["features"][lv])
                 T.append(o["choice_ms"][lv])
                 H.append(heuristic_choice(int(o["frontier"][lv]), g.n))
                 G.append(gi)
         dg.close()
-        log(f"graph {gi}: s{scale} ef{ef} a={a} n={g.n} nnz={g.nnz} "
-            f"samples={len(X)} ({time.time() - t0:.1f}s)")
+        log(f"graph {gi}: {kind} {p} n={g.n} nnz={g.nnz} samples={len(X)} ({time.time() - t0:.1f}s)")
     return (np.asarray(X, np.float32), np.asarray(T, np.float32), np.asarray(H, np.int32),
             np.asarray(G, np.int32))
 
@@ -86,6 +118,49 @@ def decision_quality(pred_choice, choice_ms, heur_choice):
             "accuracy": float(np.mean(pred_choice == choice_ms.argmin(axis=1)))}
 
 
+def validate(ctx, graphs, selectors, roots: int, reps: int, log=print):
+    """Harmonic-mean GTEPS of the deployed level loop on every validation
+    graph, for the heuristic and each selector (mean of `reps` passes over
+    the roots, wall clock)."""
+    out = {"heuristic": []}
+    for k in range(len(selectors)):
+        out[k] = []
+    edges_next = []  # (frontier out-edges / n of a push level, next level pushed), per selector
+    for kind, p in graphs:
+        dg, g = A.DeviceGraph.generate(ctx, kind, 64, 4096, 1, **p)
+        rts = g.pick_roots(roots, 17)
+        pols = [("heuristic", A.SWITCH_HEURISTIC, None)] + [(k, A.SWITCH_LEARNED, s) for k, s in
+                                                             enumerate(selectors)]
+        for name, pol, s in pols:
+            for r in rts[:2]:
+                dg.bfs(int(r), policy=pol, selector=s, results=False)  # warm-up
+            ms = []
+            for _ in range(reps):
+                dg.ctx.synchronize()
+                t0 = time.perf_counter()
+                for r in rts:
+                    dg.bfs(int(r), policy=pol, selector=s, results=False)
+                dg.ctx.synchronize()
+                ms.append((time.perf_counter() - t0) * 1e3 / len(rts))
+            out[name].append(A.gteps(g.nnz, float(np.mean(ms))))
+        for k, s in enumerate(selectors):
+            pairs = []
+            for r in rts:
+                o = dg.bfs(int(r), policy=A.SWITCH_LEARNED, selector=s, results=False)
+                ch, ent = o["choice"], o["entries"]
+                for lv in range(len(ch) - 1):
+                    if ch[lv] == A.CHOICE_PUSH:
+                        pairs.append((float(ent[lv]) / g.n, bool(ch[lv + 1] == A.CHOICE_PUSH)))
+            if len(edges_next) <= k:
+                edges_next.append([])
+            edges_next[k].extend(pairs)
+        log(f"validation {kind} {p}: " +
+            ", ".join(f"{nm}={out[nm][-1]:.1f}" for nm in out))
+        dg.close()
+    hm = {nm: A.harmonic_mean(v) for nm, v in out.items()}
+    return hm, out, edges_next
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "selector.txt"))
@@ -93,12 +168,15 @@ def main():
     ap.add_argument("--data-in", default=None, help="train from a saved dataset")
     ap.add_argument("--scales", default="18,19,20,21,22")
     ap.add_argument("--per-scale", type=int, default=5)
+    ap.add_argument("--er", type=int, default=4)
+    ap.add_argument("--grids", type=int, default=3)
     ap.add_argument("--roots", type=int, default=10)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--train-seed", type=int, default=8)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--init-seed", type=int, default=1)
+    ap.add_argument("--candidates", type=int, default=6, help="networks trained (init seeds 1..k)")
+    ap.add_argument("--val-roots", type=int, default=16)
     args = ap.parse_args()
 
     def log(*a):
@@ -110,33 +188,44 @@ def main():
         X, Tms, H, G = d["X"], d["T"], d["H"], d["G"]
     else:
         specs = graph_specs(args.train_seed, [int(s) for s in args.scales.split(",")],
-                            args.per_scale)
+                            args.per_scale, args.er, args.grids)
         X, Tms, H, G = collect(ctx, specs, args.roots, args.reps, 1000 + args.train_seed, log)
         if args.data_out:
             np.savez_compressed(args.data_out, X=X, T=Tms, H=H, G=G)
     Y = targets(Tms)
-    # hold out every fifth graph for validation
+    # held-out levels for the decision-quality report: every fifth graph
     val = (G % 5) == 4
     tr = ~val
-    sel = A.Selector(ctx, seed=args.init_seed)
-    t0 = time.time()
-    loss = sel.train(X[tr], Y[tr], steps=args.steps, lr=args.lr)
-    log(f"trained {args.steps} Adam steps on {tr.sum()} samples in {time.time() - t0:.2f}s, "
-        f"loss {loss:.5f}")
-    report = {"train_samples": int(tr.sum()), "val_samples": int(val.sum()), "train_loss": loss}
-    for name, m in (("train", tr), ("val", val)):
-        if m.sum() == 0:
-            continue
-        pred = sel.predict(X[m]).argmin(axis=1)
-        report[name] = decision_quality(pred, Tms[m], H[m])
-        report[name + "_loss"] = sel.loss_grad(X[m], Y[m])[0]
-    # final model: train on every sample
-    sel_all = A.Selector(ctx, seed=args.init_seed)
-    report["all_loss"] = sel_all.train(X, Y, steps=args.steps, lr=args.lr)
-    report["all"] = decision_quality(sel_all.predict(X).argmin(axis=1), Tms, H)
-    sel_all.save(args.out)
+    report = {"train_samples": int(tr.sum()), "heldout_samples": int(val.sum()), "candidates": []}
+    sels = []
+    for k in range(1, args.candidates + 1):
+        sel = A.Selector(ctx, seed=k)
+        t0 = time.time()
+        loss = sel.train(X[tr], Y[tr], steps=args.steps, lr=args.lr)
+        q = decision_quality(sel.predict(X[val]).argmin(axis=1), Tms[val], H[val])
+        log(f"candidate {k}: loss {loss:.5f} ({time.time() - t0:.2f}s), held-out levels "
+            f"{q['model_over_best']:.3f}x best, heuristic {q['heuristic_over_best']:.3f}x")
+        report["candidates"].append({"init_seed": k, "train_loss": loss, "heldout": q})
+        sels.append(sel)
+    hm, per_graph, edges_next = validate(ctx, VALIDATION, sels, args.val_roots, 3, log)
+    best = max(range(len(sels)), key=lambda k: hm[k])
+    for k, c in enumerate(report["candidates"]):
+        c["validation_hm_gteps"] = hm[k]
+        c["validation_gteps"] = per_graph[k]
+    report["validation_heuristic_hm_gteps"] = hm["heuristic"]
+    report["validation_graphs"] = [f"{kind} {p}" for kind, p in VALIDATION]
+    report["chosen_init_seed"] = best + 1
+    pairs = edges_next[best]
+    pushed_next = [e for e, nxt in pairs if nxt]
+    ratio = max(pushed_next) if pushed_next else 0.0
+    sels[best].queue_ratio = ratio
+    report["queue_ratio"] = ratio
+    report["queue_ratio_levels"] = {"push_then_push": len(pushed_next),
+                                    "push_then_pull": len(pairs) - len(pushed_next)}
+    sels[best].save(args.out)
     print(json.dumps(report, indent=1))
-    log(f"wrote {args.out}")
+    log(f"chose candidate {best + 1} (validation {hm[best]:.1f} GTEPS, heuristic "
+        f"{hm['heuristic']:.1f}); queue ratio {ratio:.3g}; wrote {args.out}")
 
 
 if __name__ == "__main__":

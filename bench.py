@@ -214,7 +214,7 @@ def cpu_baseline(g, roots, gpu_levels):
             "variants": var, "levels_match_gpu": parity}
 
 
-def run_batch_workload(args):
+def batch_results(ctx, count=256, steps=3, which="both"):
     """Batched multi-instance BFS: every instance in one launch.  Layouts from
     the autotuner per (family, size class); the heuristic and the learned
     direction policies on the same instances; sampled instances checked bit
@@ -223,9 +223,7 @@ def run_batch_workload(args):
     import albba_amd as A
     from albba_amd import workloads as W
 
-    count = args.instances
     specs = W.batch_specs(count)
-    ctx = A.Context(0)
     sel = None
     if os.path.exists(SELECTOR):
         sel = A.Selector(ctx)
@@ -233,8 +231,8 @@ def run_batch_workload(args):
     policies = [("heuristic", A.SWITCH_HEURISTIC, None)]
     if sel is not None:
         policies.append(("learned", A.SWITCH_LEARNED, sel))
-    if args.policy != "both":
-        policies = [p for p in policies if p[0] == args.policy]
+    if which != "both":
+        policies = [p for p in policies if p[0] == which]
     t = time.time()
     tuned = W.tune_layouts(ctx, specs, log=log)
     log(f"autotuning {time.time() - t:.1f}s")
@@ -263,7 +261,7 @@ def run_batch_workload(args):
             o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
             exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
         runs = []
-        for s in range(args.steps):
+        for s in range(steps):
             out = b.run(roots, policy=pol, selector=sl, mapping=mapping)
             runs.append(out)
             log(f"  {name} step {s}: {out['total_ms']:.2f} ms")
@@ -282,33 +280,42 @@ def run_batch_workload(args):
                      "pull_levels": int(runs[-1]["pull_levels"].sum()),
                      "push_levels": int(runs[-1]["push_levels"].sum())}
     log(f"sampled instances bit-exact: {exact}")
-    main_name = "learned" if "learned" in res else policies[0][0]
-    R = res[main_name]
-    line = {
-        "metric": f"batched BFS total GTEPS, {count} instances (RMAT s14-20, ER 2^16-2^20, "
-                  "grids 256^2-2048^2), one launch",
-        "value": R["gteps"] if exact else None, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": R["ms"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic: seeded mix (splitmix64 counter RNG), one root per instance",
-        "config": {"workload": f"batch{count}_mix{W.BATCH_MIX_SEED}", "instances": count,
-                   "rmat": kinds.count("rmat"), "er": kinds.count("er"), "grid": kinds.count("grid"),
-                   "nnz_stored_total": int(total), "policy": main_name,
-                   "layouts": {f"{k[0]}_2^{k[1]}": v for k, v in tuned.items()}},
-        "latency_ms": R["latency_ms"],
-        "policies": res,
-        "learned_over_heuristic": (round(res["learned"]["gteps"] / res["heuristic"]["gteps"], 4)
-                                   if len(res) == 2 else None),
-        "sampled_bit_exact": exact,
-    }
-    print(json.dumps(line), flush=True)
     b.close()
     for dg in dgs:
         dg.close()
     if sel is not None:
         sel.close()
+    return {"instances": count, "rmat": kinds.count("rmat"), "er": kinds.count("er"),
+            "grid": kinds.count("grid"), "nnz_stored_total": int(total),
+            "layouts": {f"{k[0]}_2^{k[1]}": v for k, v in tuned.items()}, "policies": res,
+            "learned_over_heuristic": (round(res["learned"]["gteps"] / res["heuristic"]["gteps"], 4)
+                                       if len(res) == 2 else None),
+            "sampled_bit_exact": exact}
+
+
+def run_batch_workload(args):
+    import albba_amd as A
+    from albba_amd import workloads as W
+
+    ctx = A.Context(0)
+    B = batch_results(ctx, args.instances, args.steps, args.policy)
+    main_name = "learned" if "learned" in B["policies"] else next(iter(B["policies"]))
+    R = B["policies"][main_name]
+    line = {
+        "metric": f"batched BFS total GTEPS, {B['instances']} instances (RMAT s14-20, ER 2^16-2^20, "
+                  "grids 256^2-2048^2), one launch",
+        "value": R["gteps"] if B["sampled_bit_exact"] else None, "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": R["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded mix (splitmix64 counter RNG), one root per instance",
+        "config": {"workload": f"batch{B['instances']}_mix{W.BATCH_MIX_SEED}", "policy": main_name,
+                   **{k: B[k] for k in ("instances", "rmat", "er", "grid", "nnz_stored_total", "layouts")}},
+        "latency_ms": R["latency_ms"], "policies": B["policies"],
+        "learned_over_heuristic": B["learned_over_heuristic"], "sampled_bit_exact": B["sampled_bit_exact"],
+    }
+    print(json.dumps(line), flush=True)
     ctx.close()
-    if not exact:
+    if not B["sampled_bit_exact"]:
         sys.exit(1)
 
 
@@ -426,21 +433,22 @@ def run_cpu_workload(args):
     print(json.dumps(line), flush=True)
 
 
-def run_grid_workload(args):
+def grid_results(ctx, steps=1):
     """High-diameter case: 4-neighbour grid of 4096 x 4096 vertices with 10%
     of the edges removed (seed 4, ids permuted with seed 2), 32 roots from
-    the largest component (seed 5)."""
+    the largest component (seed 5), heuristic switch, generated on the
+    device; the level loop with the persistent small-frontier kernel, and
+    (steps > 1) with one launch sequence per level for comparison."""
     import albba_amd as A
+    from albba_amd import workloads as W
 
     t = time.time()
-    g = A.Graph.grid(4096, 4096, 0.1, seed=4, perm_seed=2)
-    roots = g.pick_roots(args.roots if args.roots != 64 else 32, 5, largest_cc=True)
-    ctx = A.Context(0)
-    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+    dg, g = A.DeviceGraph.generate(ctx, "grid", 64, 4096, 1, **W.GRID4096)
+    roots = g.pick_roots(*W.GRID4096_ROOTS, largest_cc=True)
     log(f"grid 4096^2: n={g.n} nnz={g.nnz} ({time.time() - t:.1f}s)")
     res = {}
     for name, cap in (("small_frontier_kernel", None), ("per_level_launches", 0)):
-        dg.bfs(int(roots[0]), small_frontier_edges=cap, results=False)  # warm-up
+        dg.bfs(int(roots[0]), small_frontier_edges=cap, results=False, capacity=1 << 15)  # warm-up
         ms, levels = [], []
         for r in roots:
             o = dg.bfs(int(r), small_frontier_edges=cap, results=False, capacity=1 << 15)
@@ -450,21 +458,31 @@ def run_grid_workload(args):
                      "mean_ms": float(np.mean(ms)), "levels_mean": float(np.mean(levels)),
                      "us_per_level": float(np.sum(ms) * 1e3 / np.sum(levels))}
         log(f"  {name}: {res[name]}")
-        if args.steps <= 1:
+        if steps <= 1:
             break
-    R = res["small_frontier_kernel"]
+    out = {"n": g.n, "nnz_stored": g.nnz, "roots": len(roots),
+           "hm_gteps": round(res["small_frontier_kernel"]["hm_gteps"], 4),
+           "detail": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()}}
+    dg.close()
+    return out
+
+
+def run_grid_workload(args):
+    import albba_amd as A
+
+    ctx = A.Context(0)
+    R = grid_results(ctx, args.steps)
     line = {
         "metric": "harmonic-mean GTEPS, 32 roots, 4096^2 grid (10% edges removed), heuristic switch",
-        "value": round(R["hm_gteps"], 4), "unit": "GTEPS", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(R["mean_ms"] * len(roots), 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "value": R["hm_gteps"], "unit": "GTEPS", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(R["detail"]["small_frontier_kernel"]["mean_ms"] * R["roots"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic: seeded grid, permuted ids",
-        "config": {"workload": "grid4096_p0.1_32roots", "n": g.n, "nnz_stored": g.nnz,
-                   "roots": len(roots), "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
-        "detail": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()},
+        "config": {"workload": "grid4096_p0.1_32roots", "n": R["n"], "nnz_stored": R["nnz_stored"],
+                   "roots": R["roots"], "layout": "SELL-C-sigma C=64 sigma=4096 pad=1"},
+        "detail": R["detail"],
     }
     print(json.dumps(line), flush=True)
-    dg.close()
     ctx.close()
 
 
@@ -541,6 +559,8 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--roots", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="headline only: skip the grid and batch measurements")
     ap.add_argument("--workload", choices=["rmat24", "grid", "batch", "load", "cpu"], default="rmat24",
                     help="rmat24: the headline metric; grid: the 4096^2 high-diameter case; "
                          "batch: the batched multi-instance BFS; load: .cbd file to device graph; "
@@ -706,15 +726,21 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(g, roots, keep)
+    if sel is not None:
+        sel.close()
+    dg.close()
+    del g
+    if not args.no_other_configs:
+        # the high-diameter grid and the batched mix, measured in the same run
+        line["config2_grid4096"] = grid_results(ctx)
+        line["config3_batch256"] = batch_results(ctx, 256, 2)
     # a result that is not bit-exact carries no value and fails the run
-    ok = bit_exact is not False and line.get("cpu_baseline", {}).get("levels_match_gpu", True)
+    ok = (bit_exact is not False and line.get("cpu_baseline", {}).get("levels_match_gpu", True) and
+          line.get("config3_batch256", {}).get("sampled_bit_exact", True))
     if not ok:
         line["value"] = None
         line["error"] = "levels or parents differ between policies or from the oracle"
     print(json.dumps(line), flush=True)
-    if sel is not None:
-        sel.close()
-    dg.close()
     ctx.close()
     if not ok:
         sys.exit(1)

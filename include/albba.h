@@ -474,6 +474,15 @@ ALBBA_API int albba_selector_get_params(const albba_selector *s, float *params);
 ALBBA_API int albba_selector_set_params(albba_selector *s, const float *params);
 ALBBA_API int albba_selector_load(albba_selector *s, const char *path);
 ALBBA_API int albba_selector_save(const albba_selector *s, const char *path);
+/* The learned policy lists the next level's frontier queue after a push
+ * level only while that level's frontier out-edges are at most ratio * n
+ * (the next frontier is at most that large; a pull next needs no queue, a
+ * push after an unlisted level lists the bitmap).  The ratio is part of the
+ * trained selector (chosen on validation graphs from the levels where the
+ * selector pushed twice in a row) and travels in its file; 0 (unset, and
+ * every version-1 file) takes the heuristic's push threshold (int)(0.04 n). */
+ALBBA_API int albba_selector_set_queue_ratio(albba_selector *s, float ratio);
+ALBBA_API float albba_selector_queue_ratio(const albba_selector *s);
 /* Train on the GPU: x[nsamples*12] features, t[nsamples*4] targets
  * (relative log kernel times).  Adam, full batch.  Writes the final loss. */
 ALBBA_API int albba_selector_train(albba_selector *s, const float *x, const float *t,

@@ -55,6 +55,20 @@ def test_training_reduces_loss_and_round_trips(ctx, tmp_path):
     other.load(path)
     np.testing.assert_array_equal(other.params, sel.params)
     np.testing.assert_array_equal(other.predict(x), sel.predict(x))
+    assert other.queue_ratio == 0.0
+    sel.queue_ratio = 2.5e-4
+    sel.save(path)
+    other.load(path)
+    assert other.queue_ratio == np.float32(2.5e-4)
+    # a version-1 file (parameters only) loads with the ratio unset
+    lines = open(path).read().split("\n")
+    v1 = ["albba-selector 1 1604"] + lines[1:1605]
+    (tmp_path / "v1.txt").write_text("\n".join(v1) + "\n")
+    other.load(str(tmp_path / "v1.txt"))
+    assert other.queue_ratio == 0.0
+    np.testing.assert_array_equal(other.params, sel.params)
+    with pytest.raises(A.AlbbaError):
+        other.queue_ratio = -1.0
     (tmp_path / "bad.txt").write_text("not a selector\n")
     with pytest.raises(A.AlbbaError):
         other.load(str(tmp_path / "bad.txt"))
