@@ -193,6 +193,8 @@ def lib() -> C.CDLL:
         "albba_batch_free": (None, [C.c_void_p]),
         "albba_batch_run": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
                                       C.POINTER(_BatchResult), C.POINTER(C.c_float)]),
+        "albba_batch_run_noalloc": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
+                                              C.POINTER(_BatchResult), C.POINTER(C.c_float)]),
         "albba_batch_results": (C.c_int, [C.c_void_p, C.c_int32, I32P, I32P]),
         "albba_autotune": (C.c_int, [C.c_void_p, C.POINTER(_CSR), I32P, C.c_int32, C.POINTER(_Opts),
                                      C.POINTER(_TuneConfig), C.c_int32, C.POINTER(_TuneConfig),
@@ -628,8 +630,9 @@ class Batch:
         self.close()
 
     def run(self, roots, policy: int = SWITCH_HEURISTIC, selector: "Selector" = None,
-            mapping=None, workgroups: int = 0) -> dict:
-        """BFS from roots[i] on instance i; per-instance results as arrays."""
+            mapping=None, workgroups: int = 0, noalloc: bool = False) -> dict:
+        """BFS from roots[i] on instance i; per-instance results as arrays.
+        noalloc: albba_batch_run_noalloc (the run allocates no memory)."""
         n = len(self.graphs)
         r = np.ascontiguousarray(roots, dtype=np.int32)
         if r.size != n:
@@ -644,9 +647,9 @@ class Batch:
             mp = np.ascontiguousarray(np.broadcast_to(mapping, (n,)), dtype=np.int32)
         res = (_BatchResult * n)()
         ms = C.c_float()
-        _check(lib().albba_batch_run(self._p, r.ctypes.data_as(I32P), C.byref(o),
-                                     mp.ctypes.data_as(I32P) if mp is not None else None,
-                                     int(workgroups), res, C.byref(ms)))
+        fn = lib().albba_batch_run_noalloc if noalloc else lib().albba_batch_run
+        _check(fn(self._p, r.ctypes.data_as(I32P), C.byref(o),
+                  mp.ctypes.data_as(I32P) if mp is not None else None, int(workgroups), res, C.byref(ms)))
         out = {"total_ms": float(ms.value)}
         for name, _ in _BatchResult._fields_:
             out[name] = np.array([getattr(x, name) for x in res])

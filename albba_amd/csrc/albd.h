@@ -364,6 +364,13 @@ ALBD_HIDDEN int albd_batch_capacity(albd_ctx *c, int *workgroups);
 /* one launch of `workgroups` co-resident workgroups (at most
  * albd_batch_capacity) over all teams */
 ALBD_HIDDEN int albd_batch_launch(albd_ctx *c, const albd_batch_args *a, int32_t workgroups);
+/* The batched BFS's fixed device scratch (albba_batch_run_noalloc): a
+ * file-scope device array of ALBD_BATCH_SCRATCH_BYTES per device.  _init
+ * looks up its address on `device` (current) once; albd_ctx_create calls it.
+ * albd_batch_scratch returns that address for the context's device. */
+#define ALBD_BATCH_SCRATCH_BYTES ((size_t)512 << 10)
+ALBD_HIDDEN int albd_batch_scratch_init(int device);
+ALBD_HIDDEN int albd_batch_scratch(albd_ctx *c, void **ptr, size_t *bytes);
 /* ticks per millisecond of the device clock read by the batch kernel */
 ALBD_HIDDEN double albd_clock_ticks_per_ms(albd_ctx *c);
 

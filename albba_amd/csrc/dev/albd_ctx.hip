@@ -55,6 +55,8 @@ int albd_ctx_create(int device, albd_ctx **out)
     if (albd_device_count(&n) || device < 0 || device >= n)
         return albd_set_error_msg(ALBBA_ENODEV, "requested HIP device is not present");
     HIPCHK(hipSetDevice(device));
+    // the batched BFS's fixed scratch array: its address on this device
+    if (int rc = albd_batch_scratch_init(device)) return rc;
     albd_ctx *c = new albd_ctx();
     c->device = device;
     hipDeviceProp_t prop;

@@ -427,9 +427,47 @@ __global__ __launch_bounds__(kBB, 4) void k_bfs_batch(albd_batch_args a)
     }
 }
 
+// The device tables of albba_batch_run_noalloc (instance descriptors, team
+// tables) live here instead of in memory allocated per run.  Each device
+// holds its own copy at its own address; host code must use that address
+// (from hipGetSymbolAddress), never the array's name, which in host code is
+// only a stand-in.  Every run on a device shares the array: runs that
+// overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[ALBD_BATCH_SCRATCH_BYTES];
+static_assert(ALBBA_BATCH_NOALLOC_MAX * sizeof(albd_batch_item) + 256 +
+                      sizeof(int32_t) * (4096 + 3 * ALBBA_BATCH_NOALLOC_MAX + 1 + ALBBA_BATCH_NOALLOC_MAX) <=
+                  ALBD_BATCH_SCRATCH_BYTES,
+              "the scratch holds the tables of the largest no-alloc batch (up to 4096 workgroups)");
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices]; // device address of `scratch`, per device
+
 } // namespace
 
 extern "C" {
+
+int albd_batch_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return albd_set_error_msg(ALBBA_ENODEV, "batch scratch: device index beyond the table");
+    if (__atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE)) return 0;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return albd_set_error_msg(ALBBA_EHIP, "batch scratch: no device address");
+    __atomic_store_n(&g_scratch_ptr[device], p, __ATOMIC_RELEASE);
+    return 0;
+}
+
+int albd_batch_scratch(albd_ctx *c, void **ptr, size_t *bytes)
+{
+    const int device = albd_ctx_device(c);
+    void *p = device >= 0 && device < kMaxDevices ? __atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE)
+                                                  : nullptr;
+    if (!p) return albd_set_error_msg(ALBBA_EHIP, "batch scratch: not set up on this device");
+    *ptr = p;
+    *bytes = ALBD_BATCH_SCRATCH_BYTES;
+    return 0;
+}
 
 int albd_batch_capacity(albd_ctx *c, int *workgroups)
 {

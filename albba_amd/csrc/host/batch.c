@@ -220,11 +220,13 @@ static int32_t split_teams(albba_batch *b, int32_t W, int32_t *team_first, int32
 }
 
 /* One run.  The device tables (instance descriptors, team tables) go to
- * fresh device memory; the host side uses the batch's own arrays. */
+ * fresh device memory, or with `noalloc` to the library's fixed scratch
+ * array (albd_batch_scratch); the host side uses the batch's own arrays, so
+ * the noalloc form allocates nothing. */
 static int batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts, const int32_t *mapping,
-                     int32_t workgroups, albba_batch_result *results, float *total_ms)
+                     int32_t workgroups, albba_batch_result *results, float *total_ms, int noalloc)
 {
-    const char *what = "batch_run";
+    const char *what = noalloc ? "batch_run_noalloc" : "batch_run";
     if (!b || !roots) return albba_fail(ALBBA_EINVAL, "%s: bad argument", what);
     albba_bfs_opts o;
     if (opts) o = *opts;
@@ -257,7 +259,13 @@ static int batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts 
     /* device copies of the tables */
     void *dev = NULL;
     int rc = 0;
-    if ((rc = albd_malloc(d, bytes, &dev))) {
+    if (noalloc) {
+        size_t cap = 0;
+        if ((rc = albd_batch_scratch(d, &dev, &cap))) return albba_fail(rc, "%s: %s", what, albd_error());
+        if (N > ALBBA_BATCH_NOALLOC_MAX || bytes > cap)
+            return albba_fail(ALBBA_ERANGE, "%s: %d instances need %zu bytes of tables; the scratch holds "
+                              "%zu bytes, at most %d instances", what, N, bytes, cap, ALBBA_BATCH_NOALLOC_MAX);
+    } else if ((rc = albd_malloc(d, bytes, &dev))) {
         return albba_fail(rc, "%s: %s", what, albd_error());
     }
     int32_t *h_tab = (int32_t *)((char *)b->stage + b->tab_off), *p = h_tab;
@@ -290,7 +298,7 @@ static int batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts 
            (rc = albd_d2h(d, b->hctl, b->ctl, sizeof(unsigned long long) * (size_t)N * CTL_WORDS)));
     char buf[256] = "";
     if (rc) snprintf(buf, sizeof buf, "%s", albd_error());
-    albd_free(d, dev);
+    if (!noalloc) albd_free(d, dev);
     if (rc) return albba_fail(rc, "%s: %s", what, buf);
     const double tpm = albd_clock_ticks_per_ms(d);
     long long t_first = 0;
@@ -334,7 +342,14 @@ int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *
                     const int32_t *mapping, int32_t workgroups, albba_batch_result *results,
                     float *total_ms)
 {
-    return batch_run(b, roots, opts, mapping, workgroups, results, total_ms);
+    return batch_run(b, roots, opts, mapping, workgroups, results, total_ms, 0);
+}
+
+int albba_batch_run_noalloc(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
+                            const int32_t *mapping, int32_t workgroups, albba_batch_result *results,
+                            float *total_ms)
+{
+    return batch_run(b, roots, opts, mapping, workgroups, results, total_ms, 1);
 }
 
 int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent)

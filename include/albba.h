@@ -433,6 +433,19 @@ ALBBA_API void albba_batch_free(albba_batch *b);
 ALBBA_API int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
                               const int32_t *mapping, int32_t workgroups,
                               albba_batch_result *results, float *total_ms);
+/* albba_batch_run without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as albba_batch_run, except that the run's device tables go to a
+ * fixed device array of the library (512 KiB per device, set up by
+ * albba_ctx_create) instead of memory allocated per run, and batches of more
+ * than ALBBA_BATCH_NOALLOC_MAX instances are rejected (ALBBA_ERANGE).  Every
+ * call on a device shares that array: calls that overlap on one device race
+ * (one stream per context; do not run two at once). */
+#define ALBBA_BATCH_NOALLOC_MAX 1024
+ALBBA_API int albba_batch_run_noalloc(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
+                                      const int32_t *mapping, int32_t workgroups,
+                                      albba_batch_result *results, float *total_ms);
 /* level/parent of instance i after albba_batch_run (either may be NULL) */
 ALBBA_API int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent);
 
