@@ -27,8 +27,8 @@
 
 namespace {
 
-#ifndef ALBD_PULL_PIPE
-#define ALBD_PULL_PIPE 0
+#ifndef ALBD_PULL_PF0
+#define ALBD_PULL_PF0 0
 #endif
 
 constexpr int kBlock = 256;
@@ -175,10 +175,19 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     };
     int32_t w = word_of(claim());
     Head h = load_head(w);
+#if ALBD_PULL_PF0
+    // the first column of word w, loaded while the word before it ran
+    int pcol = 0;
+    bool pcol_ok = false;
+#endif
     while (w >= 0) {
         const int32_t w1 = word_of(claim());
         Head h1{0, 0, ~0ull, 0};
         bool ahead = false; // h1 loaded (wave-uniform)
+#if ALBD_PULL_PF0
+        int pcol1 = 0;
+        bool pcol1_ok = false, pf_issued = false;
+#endif
         {
             const int64_t p = (int64_t)w * 64 + lane;
             // lane row lengths replace per-column active-lane counts: rows of
@@ -200,97 +209,6 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                 const uint64_t start = unres;
                 int parent_pos = -1;
                 int j = 0, step = 1;
-#if ALBD_PULL_PIPE
-                // columns are loaded 1, 2, then 4 at a time: most lanes of a
-                // dense level hit in the first column, the survivors tend to
-                // run long.  Each step's columns are loaded right behind the
-                // previous step's frontier gathers, for the lanes still
-                // unresolved before that step, so that the column loads of
-                // one step overlap the gathers of the step before.
-                int cidx[4];
-                bool av[4];
-                auto load_cols = [&](int j0, int st, int (&ci)[4], bool (&a_)[4]) {
-                    const bool mn = (unres >> lane) & 1ull;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        a_[u] = false;
-                        ci[u] = 0;
-                        if (u < st) {
-                            const int col = j0 + u;
-                            int act;
-                            if constexpr (C <= 64)
-                                act = __popcll(__ballot(col < len) & gmask);
-                            else
-                                act = __popcll(__ballot(col < len)) + __popcll(__ballot(col < len_o));
-                            a_[u] = mn && col < len;
-                            // streaming: non-temporal, so that the frontier
-                            // and visited bitmaps stay in L2
-                            if (a_[u]) ci[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
-                            cp += (act + padm) & ~padm;
-                        }
-                    }
-                };
-                bool go = __popcll(__ballot(((unres >> lane) & 1ull) && j < len)) > g.sparse_lanes;
-                if (go) load_cols(0, 1, cidx, av);
-                while (go) {
-                    bool hit[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        bool hh = false;
-                        if (av[u]) {
-                            const int c = cidx[u];
-                            const int wd = c >> 6;
-                            bool maybe = true;
-                            if constexpr (VAR == ALBD_PULL_LIST_CB) {
-                                maybe = (lds_sum[wd >> 5] >> (wd & 31)) & 1u;
-                                cbs += maybe ? 0u : 1u;
-                            }
-                            if (maybe) hh = (a.st.front[wd] >> (c & 63)) & 1ull;
-                        }
-                        hit[u] = hh;
-                    }
-                    // the next step's columns, behind this step's gathers
-                    const int nstep = step < 4 ? 2 * step : 4;
-                    int ncidx[4];
-                    bool nav[4];
-                    const bool more = __ballot(((unres >> lane) & 1ull) && j + step < len) != 0;
-                    if (more) {
-                        load_cols(j + step, nstep, ncidx, nav);
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            nav[u] = false;
-                            ncidx[u] = 0;
-                        }
-                    }
-                    // the next word's head travels with this word's first
-                    // columns
-                    if (!ahead) {
-                        h1 = load_head(w1);
-                        ahead = true;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (u < step) {
-                            const bool hh = hit[u] && ((unres >> lane) & 1ull);
-                            if (hh) parent_pos = cidx[u];
-                            unres &= ~__ballot(hh);
-                            scanned += __popcll(__ballot(av[u]));
-                        }
-                    }
-                    j += step;
-                    step = nstep;
-                    // few live lanes left: their rows go to the row phase below
-                    go = more && unres != 0 &&
-                         __popcll(__ballot(((unres >> lane) & 1ull) && j < len)) > g.sparse_lanes;
-                    // columns loaded for lanes this step resolved are dropped
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        cidx[u] = ncidx[u];
-                        av[u] = nav[u] && ((unres >> lane) & 1ull);
-                    }
-                }
-#else
                 // columns are loaded 1, 2, then 4 at a time: most lanes of a
                 // dense level hit in the first column, the survivors tend to
                 // run long
@@ -315,7 +233,11 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             av[u] = mine && col < len;
                             // streaming: non-temporal, so that the frontier
                             // and visited bitmaps stay in L2
+#if ALBD_PULL_PF0
+                            if (av[u]) cidx[u] = col == 0 && pcol_ok ? pcol : __builtin_nontemporal_load(&g.colidx[cp + l]);
+#else
                             if (av[u]) cidx[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
+#endif
                             cp += (act + padm) & ~padm;
                         }
                     }
@@ -349,11 +271,26 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             scanned += __popcll(__ballot(av[u]));
                         }
                     }
+#if ALBD_PULL_PF0
+                    // the next word's head has arrived with this step: load
+                    // its first column now, for the lanes that will read it
+                    if (!pf_issued) {
+                        pf_issued = true;
+                        if (w1 >= 0) {
+                            int l1;
+                            if constexpr (C <= 64)
+                                l1 = (int)(((int64_t)w1 * 64 + lane) % C);
+                            else
+                                l1 = ((w1 & 1) << 6) + lane;
+                            pcol1_ok = h1.len > 0 && !((h1.vis >> lane) & 1ull);
+                            if (pcol1_ok) pcol1 = __builtin_nontemporal_load(&g.colidx[h1.cp + l1]);
+                        }
+                    }
+#endif
                     j += step;
                     if (unres == 0) break;
                     step = step < 4 ? 2 * step : 4;
                 }
-#endif
                 if (!ahead) {
                     h1 = load_head(w1);
                     ahead = true;
@@ -432,6 +369,10 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
         if (!ahead) h1 = load_head(w1);
         w = w1;
         h = h1;
+#if ALBD_PULL_PF0
+        pcol = pcol1;
+        pcol_ok = pcol1_ok;
+#endif
     }
     edges = wave_sum(edges);
     unsigned long long cb_total = 0;
@@ -439,15 +380,8 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     if (lane == 0) slot_add(a.st.slots, cnt, edges, nzw, scanned, wscan, wbyp, cb_total);
 }
 
-#if ALBD_PULL_PIPE
-// 8 waves per SIMD: two 1024-thread blocks per CU for the column-bypass
-// pull (64 VGPRs at most)
-#define ALBD_PULL_WAVES __attribute__((amdgpu_waves_per_eu(8)))
-#else
-#define ALBD_PULL_WAVES
-#endif
 template <int C, int VAR>
-__global__ __launch_bounds__(pull_block<VAR>()) ALBD_PULL_WAVES void k_pull(PullArgs a)
+__global__ __launch_bounds__(pull_block<VAR>()) void k_pull(PullArgs a)
 {
     extern __shared__ uint32_t lds_sum[];
     if (!spec_go(a.st.ctr, Spec{a.spec, a.spec_choice, ~0ull})) return;

@@ -18,12 +18,14 @@ Adam by the library's own HIP forward/backward kernels.
 
 Model selection: several networks (different initial weights) are trained
 on the training graphs' levels; each runs the deployed level loop on
-validation graphs that no benchmark or test uses (RMAT s23 and s24 and an
-Erdos-Renyi graph with seeds of their own), and the one with the highest
-harmonic-mean GTEPS over those graphs is kept.  Its queue-listing ratio
-(see albba_selector_set_queue_ratio) is the largest frontier out-edges per
-vertex at which, on the validation graphs, a push level was followed by
-another push; it is stored with the weights.
+validation graphs that no benchmark or test uses (RMAT s22, s23 and s24 and
+an Erdos-Renyi graph, with seeds of their own), and the one with the highest
+geometric mean of its speedup over the heuristic switch across those graphs
+is kept.  Its queue-listing ratio (see albba_selector_set_queue_ratio) is the
+threshold on frontier out-edges per vertex that best separates, over the
+validation graphs' push levels, those followed by another push (which need
+the queue) from those followed by a pull (which do not); it is stored with
+the weights.
 
     python -m albba_amd.train_selector --out albba_amd/selector.txt
 """
@@ -161,6 +163,21 @@ def validate(ctx, graphs, selectors, roots: int, reps: int, log=print):
     return hm, out, edges_next
 
 
+def queue_threshold(pairs):
+    """The ratio t minimizing the push levels put on the wrong side: followed
+    by a push with out-edges/n above t (queue missing), or by a pull at or
+    below t (queue listed for nothing).  Ties go to the larger t."""
+    if not pairs:
+        return 0.0
+    cand = sorted({0.0} | {e for e, _ in pairs})
+    best_t, best_err = 0.0, None
+    for t in cand:
+        err = sum(1 for e, nxt in pairs if (nxt and e > t) or (not nxt and e <= t))
+        if best_err is None or err <= best_err:
+            best_t, best_err = t, err
+    return best_t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "selector.txt"))
@@ -177,6 +194,7 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--candidates", type=int, default=6, help="networks trained (init seeds 1..k)")
     ap.add_argument("--val-roots", type=int, default=16)
+    ap.add_argument("--candidates-dir", default=None, help="also save every candidate's weights here")
     args = ap.parse_args()
 
     def log(*a):
@@ -208,24 +226,30 @@ def main():
         report["candidates"].append({"init_seed": k, "train_loss": loss, "heldout": q})
         sels.append(sel)
     hm, per_graph, edges_next = validate(ctx, VALIDATION, sels, args.val_roots, 3, log)
-    best = max(range(len(sels)), key=lambda k: hm[k])
+    heur = np.asarray(per_graph["heuristic"])
+    score = {k: float(np.exp(np.mean(np.log(np.asarray(per_graph[k]) / heur)))) for k in range(len(sels))}
+    best = max(range(len(sels)), key=lambda k: score[k])
     for k, c in enumerate(report["candidates"]):
+        c["validation_speedup_geomean"] = score[k]
         c["validation_hm_gteps"] = hm[k]
         c["validation_gteps"] = per_graph[k]
-    report["validation_heuristic_hm_gteps"] = hm["heuristic"]
+    report["validation_heuristic_gteps"] = per_graph["heuristic"]
     report["validation_graphs"] = [f"{kind} {p}" for kind, p in VALIDATION]
     report["chosen_init_seed"] = best + 1
     pairs = edges_next[best]
-    pushed_next = [e for e, nxt in pairs if nxt]
-    ratio = max(pushed_next) if pushed_next else 0.0
+    ratio = queue_threshold(pairs)
     sels[best].queue_ratio = ratio
     report["queue_ratio"] = ratio
-    report["queue_ratio_levels"] = {"push_then_push": len(pushed_next),
-                                    "push_then_pull": len(pairs) - len(pushed_next)}
+    report["queue_ratio_levels"] = {"push_then_push": sum(1 for _, nxt in pairs if nxt),
+                                    "push_then_pull": sum(1 for _, nxt in pairs if not nxt)}
+    if args.candidates_dir:
+        os.makedirs(args.candidates_dir, exist_ok=True)
+        for k, sl in enumerate(sels):
+            sl.save(os.path.join(args.candidates_dir, f"selector_{k + 1}.txt"))
     sels[best].save(args.out)
     print(json.dumps(report, indent=1))
-    log(f"chose candidate {best + 1} (validation {hm[best]:.1f} GTEPS, heuristic "
-        f"{hm['heuristic']:.1f}); queue ratio {ratio:.3g}; wrote {args.out}")
+    log(f"chose candidate {best + 1} (validation speedup over the heuristic {score[best]:.3f}, "
+        f"geometric mean); queue ratio {ratio:.3g}; wrote {args.out}")
 
 
 if __name__ == "__main__":
