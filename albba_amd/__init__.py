@@ -13,8 +13,10 @@ kernels of libalbba.so, and a missing library or GPU raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -401,18 +403,30 @@ def device_count() -> int:
     return int(n.value) if rc == 0 else 0
 
 
+# contexts still open; closed (children first) when the interpreter exits,
+# before it tears its modules down, so that nothing is left to the order in
+# which module globals disappear
+_open_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_contexts():
+    for ctx in list(_open_contexts):
+        ctx.close()
+
+
 class Context:
     """One HIP device + stream.  Objects created on it (device graphs,
-    selectors) are closed before the context itself."""
+    selectors, batches) are closed before the context itself; a context still
+    open when the interpreter exits is closed then."""
 
     def __init__(self, device: int = 0):
         self._p = C.c_void_p()
         self._children = []
         _check(lib().albba_ctx_create(device, C.byref(self._p)))
+        _open_contexts.add(self)
 
     def _adopt(self, obj):
-        import weakref
-
         self._children.append(weakref.ref(obj))
 
     def synchronize(self):
@@ -425,7 +439,8 @@ class Context:
         return float(ms.value)
 
     def close(self):
-        for ref in getattr(self, "_children", []):
+        # newest first: a batch goes before the device graphs it runs on
+        for ref in reversed(getattr(self, "_children", [])):
             obj = ref()
             if obj is not None:
                 obj.close()
@@ -433,6 +448,7 @@ class Context:
         if getattr(self, "_p", None) and self._p.value and _lib is not None:
             _lib.albba_ctx_destroy(self._p)
             self._p = C.c_void_p()
+        _open_contexts.discard(self)
 
     def __del__(self):
         self.close()
