@@ -27,10 +27,6 @@
 
 namespace {
 
-#ifndef ALBD_PULL_PF0
-#define ALBD_PULL_PF0 0
-#endif
-
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 
@@ -175,19 +171,10 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
     };
     int32_t w = word_of(claim());
     Head h = load_head(w);
-#if ALBD_PULL_PF0
-    // the first column of word w, loaded while the word before it ran
-    int pcol = 0;
-    bool pcol_ok = false;
-#endif
     while (w >= 0) {
         const int32_t w1 = word_of(claim());
         Head h1{0, 0, ~0ull, 0};
         bool ahead = false; // h1 loaded (wave-uniform)
-#if ALBD_PULL_PF0
-        int pcol1 = 0;
-        bool pcol1_ok = false, pf_issued = false;
-#endif
         {
             const int64_t p = (int64_t)w * 64 + lane;
             // lane row lengths replace per-column active-lane counts: rows of
@@ -233,11 +220,7 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             av[u] = mine && col < len;
                             // streaming: non-temporal, so that the frontier
                             // and visited bitmaps stay in L2
-#if ALBD_PULL_PF0
-                            if (av[u]) cidx[u] = col == 0 && pcol_ok ? pcol : __builtin_nontemporal_load(&g.colidx[cp + l]);
-#else
                             if (av[u]) cidx[u] = __builtin_nontemporal_load(&g.colidx[cp + l]);
-#endif
                             cp += (act + padm) & ~padm;
                         }
                     }
@@ -271,22 +254,6 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
                             scanned += __popcll(__ballot(av[u]));
                         }
                     }
-#if ALBD_PULL_PF0
-                    // the next word's head has arrived with this step: load
-                    // its first column now, for the lanes that will read it
-                    if (!pf_issued) {
-                        pf_issued = true;
-                        if (w1 >= 0) {
-                            int l1;
-                            if constexpr (C <= 64)
-                                l1 = (int)(((int64_t)w1 * 64 + lane) % C);
-                            else
-                                l1 = ((w1 & 1) << 6) + lane;
-                            pcol1_ok = h1.len > 0 && !((h1.vis >> lane) & 1ull);
-                            if (pcol1_ok) pcol1 = __builtin_nontemporal_load(&g.colidx[h1.cp + l1]);
-                        }
-                    }
-#endif
                     j += step;
                     if (unres == 0) break;
                     step = step < 4 ? 2 * step : 4;
@@ -369,10 +336,6 @@ __device__ __forceinline__ void pull_body(const PullArgs &a, uint32_t *lds_sum)
         if (!ahead) h1 = load_head(w1);
         w = w1;
         h = h1;
-#if ALBD_PULL_PF0
-        pcol = pcol1;
-        pcol_ok = pcol1_ok;
-#endif
     }
     edges = wave_sum(edges);
     unsigned long long cb_total = 0;

@@ -8,7 +8,7 @@ slice list, pull with frontier-word column bypass -- and the median kernel
 time of each choice is recorded next to the 12 level features.  Graphs are
 drawn from train_seed: RMAT of scale 18-22 with a in [0.45, 0.65]
 (b = c = (1 - a - 0.05) / 2, d = 0.05) and edge factor 8-32, Erdos-Renyi
-graphs of 2^18-2^20 vertices with mean degree 4-16, and square grids of
+graphs of 2^18-2^21 vertices with mean degree 4-32, and square grids of
 side 256-1024 with 10% of the edges removed.
 
 Targets are relative log times, t'_c = log t_c - mean_k log t_k (each time
@@ -67,7 +67,7 @@ def graph_specs(train_seed: int, scales, per_scale: int, er: int, grids: int):
     for _ in range(er):
         r = [A.rng_u64(train_seed, k * 8 + i) for i in range(4)]
         k += 1
-        specs.append(("er", dict(n=1 << (18 + int(r[0] % 3)), mean_degree=round(4 + 12 * u01(r[1]), 3),
+        specs.append(("er", dict(n=1 << (18 + int(r[0] % 4)), mean_degree=round(4 + 28 * u01(r[1]), 3),
                                  seed=int(r[2] % 1000003) + 11, perm_seed=int(r[3] % 1000003) + 13)))
     for _ in range(grids):
         r = [A.rng_u64(train_seed, k * 8 + i) for i in range(4)]
@@ -185,7 +185,7 @@ def main():
     ap.add_argument("--data-in", default=None, help="train from a saved dataset")
     ap.add_argument("--scales", default="18,19,20,21,22")
     ap.add_argument("--per-scale", type=int, default=5)
-    ap.add_argument("--er", type=int, default=4)
+    ap.add_argument("--er", type=int, default=10)
     ap.add_argument("--grids", type=int, default=3)
     ap.add_argument("--roots", type=int, default=10)
     ap.add_argument("--reps", type=int, default=3)
