@@ -188,6 +188,8 @@ def lib() -> C.CDLL:
         "albba_bfs_opts_default": (None, [C.POINTER(_Opts)]),
         "albba_bfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(_Opts),
                                 C.POINTER(_Stats)]),
+        "albba_bfs_multi": (C.c_int, [C.c_void_p, C.c_void_p, I32P, C.c_int32, C.POINTER(_Opts),
+                                      C.c_int32, C.POINTER(_Stats)]),
         "albba_bfs_results": (C.c_int, [C.c_void_p, C.c_void_p, I32P, I32P]),
         "albba_selector_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
         "albba_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
@@ -613,6 +615,38 @@ class DeviceGraph:
         if results:
             out["level"], out["parent"] = self.results()
         return out
+
+    def bfs_many(self, roots, policy: int = SWITCH_HEURISTIC, selector: "Selector" = None,
+                 scripts=None) -> dict:
+        """BFS from each root in turn in one native call (albba_bfs_multi):
+        each BFS is queued behind the previous one without a host
+        synchronisation between them.  scripts[i]: root i's per-level
+        choices (the scripted policy).  Returns per-root arrays; the graph's
+        level/parent results are the last root's."""
+        r = np.ascontiguousarray(roots, dtype=np.int32)
+        k = r.size
+        keep = []
+        opts = (_Opts * (1 if scripts is None else k))()
+        for i in range(len(opts)):
+            o = opts[i]
+            lib().albba_bfs_opts_default(C.byref(o))
+            o.switch_policy = policy
+            if selector is not None:
+                o.selector = selector._p
+            if scripts is not None:
+                scr = np.ascontiguousarray(scripts[i], dtype=np.int32)
+                keep.append(scr)
+                o.script = scr.ctypes.data_as(I32P)
+                o.script_len = scr.size
+        st = (_Stats * max(k, 1))()
+        _check(lib().albba_bfs_multi(self.ctx._p, self._p, r.ctypes.data_as(I32P), k, opts,
+                                     len(opts), st))
+        return {
+            "bfs_ms": np.array([st[i].bfs_ms for i in range(k)]),
+            "num_levels": np.array([st[i].num_levels for i in range(k)], np.int32),
+            "reached": np.array([st[i].reached for i in range(k)], np.int64),
+            "edges_in_component": np.array([st[i].edges_in_component for i in range(k)], np.int64),
+        }
 
     def results(self):
         lv = np.empty(self.n, np.int32)

@@ -161,7 +161,7 @@ ALBD_HIDDEN albd_counters *albd_host_counters(albd_ctx *c);
 ALBD_HIDDEN int albd_read_counters(albd_ctx *c, const albd_state *st); /* sync */
 
 /* ---- events (timing on the context stream) ---- */
-#define ALBD_NEVENTS 8
+#define ALBD_NEVENTS 10
 ALBD_HIDDEN int albd_event_record(albd_ctx *c, int idx);
 ALBD_HIDDEN int albd_event_elapsed(albd_ctx *c, int a, int b, float *ms); /* syncs b */
 ALBD_HIDDEN int albd_event_wait(albd_ctx *c, int idx); /* host waits for the event */
@@ -376,11 +376,18 @@ ALBD_HIDDEN double albd_clock_ticks_per_ms(albd_ctx *c);
 
 /* ---- selector MLP (mlp.hip); every pointer is device memory ---- */
 #define ALBD_MLP_SCRATCH_PER_SAMPLE (32 + 32 + 4 + 32 + 32 + 1)
+/* samples per partial sum of the gradient */
+#define ALBD_MLP_CHUNK 64
+/* floats of scratch albd_mlp_loss_grad needs for ns samples: the per-sample
+ * activations and errors, then one partial gradient (+ loss) per chunk */
+#define ALBD_MLP_SCRATCH_FLOATS(ns)                                                                \
+    ((size_t)(ns) * ALBD_MLP_SCRATCH_PER_SAMPLE +                                                  \
+     (((size_t)(ns) + ALBD_MLP_CHUNK - 1) / ALBD_MLP_CHUNK) * (ALBBA_MLP_NPARAM + 1))
 /* out[ns*4] = MLP(x[ns*12]) */
 ALBD_HIDDEN int albd_mlp_forward(albd_ctx *c, const float *params, const float *x, int32_t ns,
                                  float *out);
 /* loss[0] = mean squared error, grad[1604] = d loss / d params;
- * scratch holds ns * ALBD_MLP_SCRATCH_PER_SAMPLE floats */
+ * scratch holds ALBD_MLP_SCRATCH_FLOATS(ns) floats */
 ALBD_HIDDEN int albd_mlp_loss_grad(albd_ctx *c, const float *params, const float *x,
                                    const float *t, int32_t ns, float *scratch, float *grad,
                                    float *loss);

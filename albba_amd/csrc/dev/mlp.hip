@@ -5,10 +5,12 @@
 // Parameter layout (1604 floats): W1[32][12], b1[32], W2[32][32], b2[32],
 // W3[4][32], b3[4]; W[j][i] multiplies input i into unit j.
 //
-// The gradient is computed in two deterministic passes: one thread per
-// sample stores activations and back-propagated errors, then one thread per
-// parameter sums its sample contributions in sample order (no atomics), so a
-// run reproduces bit for bit and can be compared with a CPU restatement.
+// The gradient is computed in deterministic passes without atomics: one
+// thread per sample stores activations and back-propagated errors; a block
+// per chunk of 64 samples stages them in LDS and sums every parameter's
+// contributions over its chunk in sample order; one thread per parameter then
+// adds the chunks' partial sums in chunk order.  A run reproduces bit for bit
+// and can be compared with a CPU restatement.
 #include "dev_common.h"
 #include "../../../include/albba.h"
 
@@ -96,40 +98,71 @@ __global__ __launch_bounds__(256) void k_sample_pass(const float *params, const 
     }
 }
 
-__global__ __launch_bounds__(256) void k_param_pass(const float *x, int32_t ns,
-                                                    const float *scratch, float *grad, float *loss)
+constexpr int CH = ALBD_MLP_CHUNK;
+
+// Partial sums of chunk blockIdx.x (samples [CH*c, CH*c + m)): partial[c][p]
+// for every parameter p, partial[c][NP] the chunk's squared errors.
+__global__ __launch_bounds__(256) void k_param_partial(const float *x, int32_t ns, const float *scratch,
+                                                       float *partial)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float sh1[CH * H1], sh2[CH * H2], sdy[CH * NO], sdh1[CH * H1], sdh2[CH * H2], sx[CH * NI],
+        slv[CH];
     const float *H1s = scratch, *H2s = H1s + (size_t)ns * H1, *DY = H2s + (size_t)ns * H2,
                 *DH1 = DY + (size_t)ns * NO, *DH2 = DH1 + (size_t)ns * H1, *LV = DH2 + (size_t)ns * H2;
-    if (p == NP) {
-        float l = 0.f;
-        for (int s = 0; s < ns; ++s) l += LV[s];
-        *loss = l / ((float)ns * (float)NO);
-        return;
+    const int c = blockIdx.x;
+    const size_t s0 = (size_t)c * CH;
+    const int m = (int)((size_t)ns - s0 < (size_t)CH ? (size_t)ns - s0 : (size_t)CH);
+    for (int i = threadIdx.x; i < m * H1; i += blockDim.x) {
+        sh1[i] = H1s[s0 * H1 + i];
+        sdh1[i] = DH1[s0 * H1 + i];
     }
+    for (int i = threadIdx.x; i < m * H2; i += blockDim.x) {
+        sh2[i] = H2s[s0 * H2 + i];
+        sdh2[i] = DH2[s0 * H2 + i];
+    }
+    for (int i = threadIdx.x; i < m * NO; i += blockDim.x) sdy[i] = DY[s0 * NO + i];
+    for (int i = threadIdx.x; i < m * NI; i += blockDim.x) sx[i] = x[s0 * NI + i];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) slv[i] = LV[s0 + i];
+    __syncthreads();
+    float *out = partial + (size_t)c * (NP + 1);
+    for (int p = threadIdx.x; p <= NP; p += blockDim.x) {
+        float g = 0.f;
+        if (p == NP) {
+            for (int q = 0; q < m; ++q) g += slv[q];
+        } else if (p < OB1) {
+            const int j = (p - OW1) / NI, i = (p - OW1) % NI;
+            for (int q = 0; q < m; ++q) g = fmaf(sdh1[q * H1 + j], sx[q * NI + i], g);
+        } else if (p < OW2) {
+            const int j = p - OB1;
+            for (int q = 0; q < m; ++q) g += sdh1[q * H1 + j];
+        } else if (p < OB2) {
+            const int j = (p - OW2) / H1, i = (p - OW2) % H1;
+            for (int q = 0; q < m; ++q) g = fmaf(sdh2[q * H2 + j], sh1[q * H1 + i], g);
+        } else if (p < OW3) {
+            const int j = p - OB2;
+            for (int q = 0; q < m; ++q) g += sdh2[q * H2 + j];
+        } else if (p < OB3) {
+            const int k = (p - OW3) / H2, i = (p - OW3) % H2;
+            for (int q = 0; q < m; ++q) g = fmaf(sdy[q * NO + k], sh2[q * H2 + i], g);
+        } else {
+            const int k = p - OB3;
+            for (int q = 0; q < m; ++q) g += sdy[q * NO + k];
+        }
+        out[p] = g;
+    }
+}
+
+// grad[p] = the chunks' partial sums in chunk order; loss = the mean of the
+// squared errors
+__global__ __launch_bounds__(256) void k_param_reduce(const float *partial, int32_t nchunks, int32_t ns,
+                                                      float *grad, float *loss)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > NP) return;
     float g = 0.f;
-    if (p < OB1) {
-        const int j = (p - OW1) / NI, i = (p - OW1) % NI;
-        for (int s = 0; s < ns; ++s) g = fmaf(DH1[(size_t)s * H1 + j], x[(size_t)s * NI + i], g);
-    } else if (p < OW2) {
-        const int j = p - OB1;
-        for (int s = 0; s < ns; ++s) g += DH1[(size_t)s * H1 + j];
-    } else if (p < OB2) {
-        const int j = (p - OW2) / H1, i = (p - OW2) % H1;
-        for (int s = 0; s < ns; ++s) g = fmaf(DH2[(size_t)s * H2 + j], H1s[(size_t)s * H1 + i], g);
-    } else if (p < OW3) {
-        const int j = p - OB2;
-        for (int s = 0; s < ns; ++s) g += DH2[(size_t)s * H2 + j];
-    } else if (p < OB3) {
-        const int k = (p - OW3) / H2, i = (p - OW3) % H2;
-        for (int s = 0; s < ns; ++s) g = fmaf(DY[(size_t)s * NO + k], H2s[(size_t)s * H2 + i], g);
-    } else {
-        const int k = p - OB3;
-        for (int s = 0; s < ns; ++s) g += DY[(size_t)s * NO + k];
-    }
-    grad[p] = g;
+    for (int c = 0; c < nchunks; ++c) g += partial[(size_t)c * (NP + 1) + p];
+    if (p == NP) *loss = g / ((float)ns * (float)NO);
+    else grad[p] = g;
 }
 
 __global__ __launch_bounds__(256) void k_adam(float *params, const float *grad, float *m, float *v,
@@ -165,7 +198,12 @@ int albd_mlp_loss_grad(albd_ctx *c, const float *params, const float *x, const f
     hipStream_t s = albd_stream(c);
     hipLaunchKernelGGL(k_sample_pass, dim3((ns + 255) / 256), dim3(256), 0, s, params, x, t, ns, scratch);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_param_pass, dim3((NP + 1 + 255) / 256), dim3(256), 0, s, x, ns, scratch, grad, loss);
+    const int32_t nchunks = (ns + CH - 1) / CH;
+    float *partial = scratch + (size_t)ns * ALBD_MLP_SCRATCH_PER_SAMPLE;
+    hipLaunchKernelGGL(k_param_partial, dim3(nchunks), dim3(256), 0, s, x, ns, scratch, partial);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_param_reduce, dim3((NP + 1 + 255) / 256), dim3(256), 0, s, partial, nchunks, ns, grad,
+                       loss);
     HIPCHK(hipGetLastError());
     return 0;
 }

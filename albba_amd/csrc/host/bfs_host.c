@@ -919,13 +919,20 @@ static int cmp_float(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
+/* events of a BFS: its start and end (a second pair lets the next BFS of a
+ * sequence start before the previous one's time is read) */
+typedef struct bfs_events {
+    int start, end;
+    int defer; /* leave bfs_ms to the caller, which reads the pair later */
+} bfs_events;
+
 static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
-                   albba_bfs_stats *stats);
+                   albba_bfs_stats *stats, bfs_events ev);
 
 /* pmin and the small-frontier keys are all ones at rest (the kernels restore
  * what they use); after a failed BFS they are refilled before the next one */
-int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
-              albba_bfs_stats *stats)
+static int bfs_checked(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
+                       albba_bfs_stats *stats, bfs_events ev)
 {
     if (ctx && G && G->ctx == ctx && G->needs_reset) {
         int rc = albd_fill_u32(ctx->d, G->st.pmin, 0xFFFFFFFFu, (size_t)G->g.npos);
@@ -934,8 +941,51 @@ int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts
         if (rc) return albba_fail(rc, "bfs: %s", albd_error());
         G->needs_reset = 0;
     }
-    int rc = bfs_run(ctx, G, root, opts_in, stats);
+    int rc = bfs_run(ctx, G, root, opts_in, stats, ev);
     if (rc && rc != ALBBA_EINVAL && G) G->needs_reset = 1;
+    return rc;
+}
+
+int albba_bfs(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
+              albba_bfs_stats *stats)
+{
+    const bfs_events ev = {0, 1, 0};
+    return bfs_checked(ctx, G, root, opts_in, stats, ev);
+}
+
+int albba_bfs_multi(albba_ctx *ctx, albba_graph *G, const int32_t *roots, int32_t nroots,
+                    const albba_bfs_opts *opts, int32_t nopts, albba_bfs_stats *stats)
+{
+    if (!ctx || !G || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "bfs_multi: bad context/graph");
+    if (nroots < 0 || (nroots > 0 && !roots))
+        return albba_fail(ALBBA_EINVAL, "bfs_multi: %d roots", nroots);
+    if (nopts != 1 && nopts != nroots)
+        return albba_fail(ALBBA_EINVAL, "bfs_multi: nopts %d must be 1 or nroots %d", nopts, nroots);
+    if (!opts) return albba_fail(ALBBA_EINVAL, "bfs_multi: no options");
+    for (int32_t i = 0; i < nroots; ++i)
+        if (roots[i] < 0 || roots[i] >= G->n)
+            return albba_fail(ALBBA_EINVAL, "bfs_multi: root %d out of range [0,%d)", roots[i], G->n);
+    albd_ctx *d = ctx->d;
+    int rc = 0;
+    for (int32_t i = 0; i < nroots && !rc; ++i) {
+        /* BFS i uses pair i % 2; its level loop waited on mailboxes written
+         * after BFS i-1 ended (one stream), so that pair is complete */
+        const bfs_events ev = {(i & 1) ? 8 : 0, (i & 1) ? 9 : 1, 1};
+        rc = bfs_checked(ctx, G, roots[i], nopts == 1 ? opts : opts + i, stats ? stats + i : NULL, ev);
+        if (!rc && i > 0 && stats) {
+            const int s = (i & 1) ? 0 : 8;
+            float ms = 0;
+            if ((rc = albd_event_elapsed(d, s, s + 1, &ms))) rc = albba_fail(rc, "bfs_multi: %s", albd_error());
+            stats[i - 1].bfs_ms = ms;
+        }
+    }
+    if (!rc && nroots > 0) {
+        const int s = ((nroots - 1) & 1) ? 8 : 0;
+        float ms = 0;
+        if (stats && !(rc = albd_event_elapsed(d, s, s + 1, &ms))) stats[nroots - 1].bfs_ms = ms;
+        if (!rc) rc = albd_sync(d);
+        if (rc) rc = albba_fail(rc, "bfs_multi: %s", albd_error());
+    }
     return rc;
 }
 
@@ -949,7 +999,7 @@ static int script_at(const albba_graph *G, const albba_bfs_opts *o, int32_t leve
 }
 
 static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs_opts *opts_in,
-                   albba_bfs_stats *stats)
+                   albba_bfs_stats *stats, bfs_events ev)
 {
     if (!ctx || !G || G->ctx != ctx) return albba_fail(ALBBA_EINVAL, "bfs: bad context/graph");
     if (root < 0 || root >= G->n) return albba_fail(ALBBA_EINVAL, "bfs: root %d out of range [0,%d)", root, G->n);
@@ -1006,7 +1056,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
      * queued, or -1; whether the push kernels are */
     int queued_pull = -1, queued_push = 0;
     /* init, then the select kernel turns the root into level 0's frontier */
-    if ((rc = albd_event_record(d, 0)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
+    if ((rc = albd_event_record(d, ev.start)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (spec)
         rc = select_prep(G, &ls, &sel, &seq) ||
@@ -1204,14 +1254,14 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         level += nlev;
     }
     float ms = 0;
-    if ((rc = albd_to_vertex_order(d, &G->g, &G->st)) || (rc = albd_event_record(d, 1)) ||
-        (rc = albd_event_elapsed(d, 0, 1, &ms)))
+    if ((rc = albd_to_vertex_order(d, &G->g, &G->st)) || (rc = albd_event_record(d, ev.end)) ||
+        (!ev.defer && (rc = albd_event_elapsed(d, ev.start, ev.end, &ms))))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (stats) {
         stats->num_levels = level;
         stats->reached = (int64_t)hc->reached;
         stats->edges_in_component = (int64_t)hc->reached_edges;
-        stats->bfs_ms = ms;
+        if (!ev.defer) stats->bfs_ms = ms;
     }
     return ALBBA_OK;
 }

@@ -150,7 +150,7 @@ static int batch_alloc(albd_ctx *d, int32_t ns, int with_t, int with_train, batc
     memset(b, 0, sizeof(*b));
     int rc = albd_malloc(d, sizeof(float) * (size_t)ns * ALBBA_MLP_IN, (void **)&b->x);
     if (!rc && with_t) rc = albd_malloc(d, sizeof(float) * (size_t)ns * ALBBA_MLP_OUT, (void **)&b->t);
-    if (!rc && with_t) rc = albd_malloc(d, sizeof(float) * (size_t)ns * ALBD_MLP_SCRATCH_PER_SAMPLE, (void **)&b->scratch);
+    if (!rc && with_t) rc = albd_malloc(d, sizeof(float) * ALBD_MLP_SCRATCH_FLOATS(ns), (void **)&b->scratch);
     if (!rc && with_t) rc = albd_malloc(d, sizeof(float) * ALBBA_MLP_NPARAM, (void **)&b->grad);
     if (!rc && with_t) rc = albd_malloc(d, sizeof(float) * 4, (void **)&b->loss);
     if (!rc && with_train) rc = albd_malloc(d, sizeof(float) * ALBBA_MLP_NPARAM, (void **)&b->m);

@@ -57,12 +57,11 @@ def run_policy(dg, roots, policy, sel, steps, scripts=None):
     for s in range(steps):
         dg.ctx.synchronize()
         t0 = time.perf_counter()
-        for i, r in enumerate(roots):
-            o = dg.bfs(int(r), policy=policy, selector=sel, results=False,
-                       script=None if scripts is None else scripts[i])
-            per_root[i] += o["bfs_ms"] / steps
-            comp[i] = o["edges_in_component"]
+        # one native call runs the roots back to back (albba_bfs_multi)
+        o = dg.bfs_many(roots, policy=policy, selector=sel, scripts=scripts)
         dg.ctx.synchronize()
+        per_root += o["bfs_ms"] / steps
+        comp[:] = o["edges_in_component"]
         step_ms.append((time.perf_counter() - t0) * 1e3)
         log(f"  step {s}: {step_ms[-1]:.1f} ms")
     return per_root, float(np.mean(step_ms)), comp

@@ -392,6 +392,14 @@ ALBBA_API void albba_bfs_opts_default(albba_bfs_opts *o);
  * albba_bfs_results copies them out.  stats may be NULL. */
 ALBBA_API int albba_bfs(albba_ctx *ctx, albba_graph *g, int32_t root,
                         const albba_bfs_opts *opts, albba_bfs_stats *stats);
+/* BFS from roots[0..nroots) in turn on one graph (Graph500's sequence of
+ * roots; the reference runs one root per process, bfs/src/main.c:196-213).
+ * Each BFS's first kernels are queued while the previous one's last kernels
+ * still run: one synchronisation at the end instead of one per root.  opts:
+ * one set for every root (nopts = 1) or one per root (nopts = nroots); stats:
+ * nroots entries, or NULL.  The graph's results are the last root's. */
+ALBBA_API int albba_bfs_multi(albba_ctx *ctx, albba_graph *g, const int32_t *roots, int32_t nroots,
+                              const albba_bfs_opts *opts, int32_t nopts, albba_bfs_stats *stats);
 /* level[v]: BFS distance or -1; parent[v]: min-id parent, root for the root,
  * -1 if unreached.  Either pointer may be NULL. */
 ALBBA_API int albba_bfs_results(albba_ctx *ctx, albba_graph *g, int32_t *level,

@@ -291,3 +291,34 @@ def test_bypass_counters(ctx):
     push = check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PUSH)
     assert (push["bypassed"] == 0).all() and (push["col_bypassed"] == 0).all()
     dg.close()
+
+
+def test_bfs_multi_matches_single_runs(ctx):
+    """albba_bfs_multi runs the roots back to back without a host
+    synchronisation between them: every root's statistics equal those of a
+    BFS of its own, the graph's results are the last root's (against the
+    oracle), and per-root scripts are honoured."""
+    g = A.Graph.rmat(15, 16, seed=131, perm_seed=132)
+    rp, col = g.row_ptr.copy(), g.col.copy()
+    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+    roots = [int(r) for r in g.pick_roots(7, 133)]
+    rng = np.random.default_rng(134)
+    scripts = [rng.integers(0, 4, 12).astype(np.int32) for _ in roots]
+    try:
+        for pol, scr in ((A.SWITCH_HEURISTIC, None), (A.SWITCH_SCRIPT, scripts)):
+            many = dg.bfs_many(roots, policy=pol, scripts=scr)
+            lv, pa = dg.results()
+            wl, wp = O.bfs_queue(rp, col, roots[-1])
+            np.testing.assert_array_equal(lv, wl)
+            np.testing.assert_array_equal(pa, wp)
+            for i, r in enumerate(roots):
+                one = dg.bfs(r, policy=pol, script=None if scr is None else scr[i], results=False)
+                for k in ("num_levels", "reached", "edges_in_component"):
+                    assert many[k][i] == one[k], (k, i)
+            assert (many["bfs_ms"] > 0).all()
+        assert dg.bfs_many([])["bfs_ms"].size == 0
+        with pytest.raises(A.AlbbaError) as e:
+            dg.bfs_many([roots[0], g.n])
+        assert e.value.code == -1
+    finally:
+        dg.close()

@@ -144,3 +144,20 @@ def test_config3_batch256_every_instance(ctx):
         for dg in dgs:
             dg.close()
         sel.close()
+
+
+def test_column_bypass_past_2_24_vertices(ctx):
+    """The column-bypass pull's LDS frontier summary (a bit per 64-vertex
+    word, 34 KB here) on an ER graph of 2^24 + 2^20 vertices, every level on
+    that pull and under the heuristic, against the oracle's queue BFS."""
+    n = (1 << 24) + (1 << 20)
+    dg, g = A.DeviceGraph.generate(ctx, "er", 64, 4096, 1, n=n, mean_degree=4.0, seed=501, perm_seed=502)
+    try:
+        root = int(g.pick_roots(1, 503)[0])
+        lv, pa = O.bfs_queue(g.row_ptr, g.col, root)
+        for pol in (A.SWITCH_FIXED, A.SWITCH_HEURISTIC):
+            o = dg.bfs(root, policy=pol, fixed_choice=A.CHOICE_PULL_CB)
+            np.testing.assert_array_equal(o["level"], lv)
+            np.testing.assert_array_equal(o["parent"], pa)
+    finally:
+        dg.close()

@@ -22,8 +22,10 @@ def batch(ns, seed):
     return x, t
 
 
-@pytest.mark.parametrize("ns", [1, 7, 300, 2049])
+@pytest.mark.parametrize("ns", [1, 7, 64, 300, 2049, 40000])
 def test_gpu_loss_and_gradient_match_oracle(ctx, ns):
+    """Sizes below, at and across the 64-sample chunks of the gradient's
+    partial sums (ragged last chunks of 44 and 1 samples)."""
     sel = A.Selector(ctx, seed=ns)
     x, t = batch(ns, ns)
     loss, grad = sel.loss_grad(x, t)
@@ -32,6 +34,10 @@ def test_gpu_loss_and_gradient_match_oracle(ctx, ns):
     # float32 kernels vs float64 oracle
     assert abs(loss - rl) <= 1e-5 * abs(rl) + 1e-7
     np.testing.assert_allclose(grad, rg, rtol=2e-4, atol=2e-6 * np.abs(rg).max())
+    # no atomics: a second call gives the same bits
+    loss2, grad2 = sel.loss_grad(x, t)
+    assert loss2 == loss
+    np.testing.assert_array_equal(grad2, grad)
     y = sel.predict(x)
     np.testing.assert_allclose(y, M.forward(p, x)[0], rtol=1e-5, atol=1e-5)
     sel.close()

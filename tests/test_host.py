@@ -396,3 +396,20 @@ def test_pick_roots():
     assert len(set(sizes)) == 1
     assert tc.pick_roots(1000, 1).size == int((tc.degrees() > 0).sum())
     assert big  # component of vertex 0 exists
+
+
+def test_selector_queue_threshold():
+    """The trainer's queue-listing ratio: the out-edges-per-vertex threshold
+    that misplaces the fewest push levels (followed by a push above it, or by
+    a pull at or below it), ties to the larger threshold."""
+    from albba_amd.train_selector import queue_threshold
+
+    assert queue_threshold([]) == 0.0
+    sep = [(0.001, True), (0.002, True), (0.01, False), (0.02, False)]
+    assert queue_threshold(sep) == 0.002
+    # one misplaced level either way at 0 and at 0.002: the larger wins
+    assert queue_threshold([(0.001, False), (0.002, True)]) == 0.002
+    # every push followed by a pull: nothing needs listing
+    assert queue_threshold([(0.003, False), (0.004, False)]) == 0.0
+    # every push followed by a push: list up to the largest
+    assert queue_threshold([(0.003, True), (0.004, True)]) == 0.004
