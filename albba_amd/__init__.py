@@ -99,7 +99,7 @@ class _Stats(C.Structure):
                 ("edges_in_component", C.c_int64), ("bfs_ms", C.c_double), ("frontier", I32P),
                 ("choice", I32P), ("level_ms", F32P), ("features", F32P), ("choice_ms", F32P),
                 ("entries", I64P), ("words", I64P), ("bypassed", I64P), ("kernel_ms", F32P),
-                ("col_bypassed", I64P)]
+                ("col_bypassed", I64P), ("swept", I64P)]
 
 
 class _BatchResult(C.Structure):
@@ -195,6 +195,7 @@ def lib() -> C.CDLL:
         "albba_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32,
                                          C.POINTER(C.c_void_p)]),
         "albba_batch_free": (None, [C.c_void_p]),
+        "albba_batch_forget": (None, [C.c_void_p]),
         "albba_batch_run": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
                                       C.POINTER(_BatchResult), C.POINTER(C.c_float)]),
         "albba_batch_run_noalloc": (C.c_int, [C.c_void_p, I32P, C.POINTER(_Opts), I32P, C.c_int32,
@@ -597,10 +598,11 @@ class DeviceGraph:
         by = np.zeros(cap, np.int64)
         km = np.zeros(cap, np.float32)
         cb = np.zeros(cap, np.int64)
+        sw = np.full(cap, -1, np.int64)
         st = _Stats(0, cap, 0, 0, 0.0, fr.ctypes.data_as(I32P), ch.ctypes.data_as(I32P),
                     lm.ctypes.data_as(F32P), ft.ctypes.data_as(F32P), cm.ctypes.data_as(F32P),
                     en.ctypes.data_as(I64P), wd.ctypes.data_as(I64P), by.ctypes.data_as(I64P),
-                    km.ctypes.data_as(F32P), cb.ctypes.data_as(I64P))
+                    km.ctypes.data_as(F32P), cb.ctypes.data_as(I64P), sw.ctypes.data_as(I64P))
         _check(lib().albba_bfs(self.ctx._p, self._p, int(root), C.byref(o), C.byref(st)))
         L = min(st.num_levels, cap)
         out = {
@@ -611,6 +613,8 @@ class DeviceGraph:
             "choice_ms": cm[:L * 4].reshape(L, 4).copy(), "entries": en[:L].copy(),
             "words": wd[:L].copy(), "bypassed": by[:L].copy(), "kernel_ms": km[:L].copy(),
             "col_bypassed": cb[:L].copy(),
+            # stored entries of the words a list pull scanned (time_levels)
+            "swept": sw[:L].copy(),
         }
         if results:
             out["level"], out["parent"] = self.results()
@@ -704,6 +708,11 @@ class Batch:
         for name, _ in _BatchResult._fields_:
             out[name] = np.array([getattr(x, name) for x in res])
         return out
+
+    def forget(self):
+        """Drop the previous run's levels and times: the next run splits the
+        workgroups by the instances' shares of the stored entries."""
+        lib().albba_batch_forget(self._p)
 
     def results(self, i: int):
         g = self.graphs[i]

@@ -252,6 +252,8 @@ enum { ALBD_PULL_ALL = 0, ALBD_PULL_LIST = 1, ALBD_PULL_LIST_CB = 2 };
 ALBD_HIDDEN int albd_build_active(albd_ctx *c, const albd_graph *g, albd_state *st,
                                   int from_list, int64_t prev_len, int32_t level);
 ALBD_HIDDEN int albd_build_summary(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* out[nwords] (device) = stored entries of every position word */
+ALBD_HIDDEN int albd_word_entries(albd_ctx *c, const albd_graph *g, int64_t *out);
 /* As albd_pull, queued before the level's direction is known: the kernel
  * returns at once unless the select kernel chose this variant (spec = 1: its
  * learned choice, 0: the heuristic's) and the frontier is not empty; spec =

@@ -260,11 +260,33 @@ static __global__ __launch_bounds__(256) void k_copy_u64(ulonglong2 *dst, const 
         dst[i] = src[i];
 }
 
+// out[w] = the stored entries of position word w (its 64 row lengths
+// summed): the column entries a full scan of the word's slices reads
+static __global__ __launch_bounds__(256) void k_word_entries(const int32_t *row_len, int64_t nwords,
+                                                             int64_t *out)
+{
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = gt >> 6;
+    if (w >= nwords) return; // whole waves: 64 lanes share w
+    const unsigned long long s = wave_sum((unsigned long long)row_len[gt]);
+    if ((threadIdx.x & 63) == 0) out[w] = (int64_t)s;
+}
+
 static unsigned fill_blocks(albd_ctx *c, size_t n)
 {
     size_t b = (n + 255) / 256;
     size_t cap = (size_t)c->ncu * 16;
     return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+int albd_word_entries(albd_ctx *c, const albd_graph *g, int64_t *out)
+{
+    const int64_t threads = (int64_t)g->nwords * 64;
+    if (threads <= 0) return 0;
+    hipLaunchKernelGGL(k_word_entries, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                       g->row_len, (int64_t)g->nwords, out);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int albd_fill_u32(albd_ctx *c, void *p, uint32_t v, size_t count)

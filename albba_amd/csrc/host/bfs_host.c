@@ -172,6 +172,8 @@ void albba_graph_free(albba_graph *G)
     albd_free(d, G->scratch_ctr);
     free(G->pos_of);
     free(G->host_log);
+    free(G->word_entries);
+    free(G->list_copy);
     free(G);
 }
 
@@ -809,6 +811,46 @@ static int select_prep(albba_graph *G, const loop_state *ls, const albd_select_a
 /* The kernels of one level after prep_level (no pointer swaps); a push
  * lists the next frontier's queue while its frontier edges are at most
  * build_max. */
+/* The stored entries of the words the list pull of `level` scanned: the
+ * column entries a full scan of those slices reads.  The pull's word list
+ * and its length are copied back (after the pull, before the next level
+ * rebuilds them) and summed over a per-word table made once per graph.  For
+ * the per-level accounting only (time_levels). */
+static int swept_entries(albba_graph *G, int32_t level, int64_t *out)
+{
+    albd_ctx *d = G->ctx->d;
+    const int64_t nw = G->g.nwords;
+    int rc;
+    if (!G->word_entries) {
+        int64_t *dev = NULL;
+        G->word_entries = (int64_t *)albba_malloc(sizeof(int64_t) * (size_t)nw);
+        G->list_copy = (int32_t *)albba_malloc(sizeof(int32_t) * (size_t)nw);
+        rc = !G->word_entries || !G->list_copy ? ALBBA_ENOMEM : 0;
+        if (!rc && !(rc = albd_malloc(d, sizeof(int64_t) * (size_t)nw, (void **)&dev)) &&
+            !(rc = albd_word_entries(d, &G->g, dev)))
+            rc = albd_d2h(d, G->word_entries, dev, sizeof(int64_t) * (size_t)nw);
+        albd_free(d, dev);
+        if (rc) {
+            free(G->word_entries);
+            free(G->list_copy);
+            G->word_entries = NULL;
+            G->list_copy = NULL;
+            return rc == ALBBA_ENOMEM ? albba_fail(rc, "bfs: swept entries: no memory")
+                                      : albba_fail(rc, "bfs: swept entries: %s", albd_error());
+        }
+    }
+    unsigned long long cnt = 0;
+    if ((rc = albd_d2h(d, &cnt, &G->st.ctr->list_cnt[level % 3], sizeof cnt)))
+        return albba_fail(rc, "bfs: swept entries: %s", albd_error());
+    if (cnt > (unsigned long long)nw) cnt = (unsigned long long)nw;
+    if (cnt && (rc = albd_d2h(d, G->list_copy, G->st.active_next, sizeof(int32_t) * (size_t)cnt)))
+        return albba_fail(rc, "bfs: swept entries: %s", albd_error());
+    int64_t sum = 0;
+    for (unsigned long long k = 0; k < cnt; ++k) sum += G->word_entries[G->list_copy[k]];
+    *out = sum;
+    return 0;
+}
+
 static int run_main(albba_graph *G, const loop_state *ls, int choice, int32_t level,
                     int64_t frontier, int mark, int64_t build_max)
 {
@@ -1150,6 +1192,11 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
             rc = run_level(G, &ls, choice, level, frontier, opts.time_levels);
         if (rc) return albba_fail(rc, "bfs: %s", albd_error());
         if (opts.time_levels && (rc = albd_event_record(d, 3))) return albba_fail(rc, "bfs: %s", albd_error());
+        int64_t swept = -1;
+        if (opts.time_levels && stats && stats->swept && level < stats->capacity && !small &&
+            (choice == ALBBA_CHOICE_PULL_SB || choice == ALBBA_CHOICE_PULL_CB) &&
+            (rc = swept_entries(G, level, &swept)))
+            return rc;
         if (small && nlev > 1 && stats &&
             (rc = albd_d2h(d, G->host_log, G->st.level_log, sizeof(int64_t) * 4 * (size_t)(nlev - 1))))
             return albba_fail(rc, "bfs: %s", albd_error());
@@ -1246,6 +1293,7 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                  * or on it but visited or empty by now */
                 if (stats->bypassed) stats->bypassed[lv] = push || small ? 0 : (int64_t)G->g.nwords - sums.words;
                 if (stats->col_bypassed) stats->col_bypassed[lv] = push || small ? 0 : sums.col_bypassed;
+                if (stats->swept) stats->swept[lv] = j == 0 ? swept : -1;
             }
         }
         if (!spec) commit_pointers(G, &ls, choice, 1);

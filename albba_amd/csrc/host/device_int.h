@@ -24,6 +24,9 @@ struct albba_graph {
     void *scratch_visited; /* BEST-policy snapshots */
     void *scratch_ctr;
     int64_t *host_log;     /* small-frontier level sizes, 4 per level */
+    int64_t *word_entries; /* host: stored entries per position word, made
+                            * on the first BFS that accounts swept entries */
+    int32_t *list_copy;    /* host: a pull's word list, for that accounting */
     int needs_reset;       /* a BFS failed midway: pmin/pkey may be dirty */
 };
 

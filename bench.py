@@ -30,9 +30,12 @@ def log(*a):
 
 
 def pull_bytes(entries, words, n, n_new):
-    # 4 B per gathered column entry, 8 B slice metadata + 2 x (C/8) B visited
-    # read/write per scanned 64-lane word, one frontier-bitmap pass, 8 B of
-    # level+parent per discovered vertex
+    # 4 B per column entry of the scanned (not bypassed) slices -- every
+    # stored entry of the words scanned, the entries a full scan reads --,
+    # 8 B slice metadata + 2 x (C/8) B visited read/write per scanned 64-lane
+    # word, one frontier-bitmap pass, 8 B of level+parent per discovered
+    # vertex.  With `entries` = the entries the early-exit scan gathered, the
+    # same formula gives the bytes this kernel cannot avoid.
     return 4 * entries + 8 * words + 16 * words + n / 8 + 8 * n_new
 
 
@@ -79,11 +82,19 @@ def account_kernels(dg, roots, policy, sel, n):
         for lv in range(L):
             n_new = o["frontier"][lv + 1] if lv + 1 < L else 0
             ch = int(o["choice"][lv])
-            b = (push_bytes(o["words"][lv], o["entries"][lv], n, n_new) if ch == A.CHOICE_PUSH
-                 else pull_bytes(o["entries"][lv], o["words"][lv], n, n_new))
-            p = prof.setdefault(KERNEL_OF_CHOICE[ch], {"ms": 0.0, "bytes": 0.0, "launches": 0})
+            if ch == A.CHOICE_PUSH:
+                b = bg = push_bytes(o["words"][lv], o["entries"][lv], n, n_new)
+            else:
+                # the scanned words' stored entries where the library counted
+                # them (list pulls), else the gathered ones
+                sw = o["swept"][lv] if o["swept"][lv] >= 0 else o["entries"][lv]
+                b = pull_bytes(sw, o["words"][lv], n, n_new)
+                bg = pull_bytes(o["entries"][lv], o["words"][lv], n, n_new)
+            p = prof.setdefault(KERNEL_OF_CHOICE[ch], {"ms": 0.0, "bytes": 0.0, "bytes_gathered": 0.0,
+                                                       "launches": 0})
             p["ms"] += float(o["kernel_ms"][lv])
             p["bytes"] += float(b)
+            p["bytes_gathered"] += float(bg)
             p["launches"] += 1
     return prof
 
@@ -237,11 +248,14 @@ def batch_results(ctx, count=256, steps=3, which="both"):
     log(f"autotuning {time.time() - t:.1f}s")
     t = time.time()
     dgs, roots, nnz, kinds, keep = [], [], [], [], {}
+    other_roots = [[] for _ in range(steps)]
     for i, (kind, p) in enumerate(specs):
         cfg = W.tuned_config(tuned, kind, p)
         dg, g = W.generate_instance(ctx, kind, p, cfg)
         dgs.append(dg)
         roots.append(W.root_of(kind, g, i))
+        for k in range(steps):  # other queries on the same graphs
+            other_roots[k].append(W.root_of(kind, g, i + (k + 1) * count))
         nnz.append(g.nnz)
         kinds.append(kind)
         del g
@@ -254,22 +268,37 @@ def batch_results(ctx, count=256, steps=3, which="both"):
     total = float(np.sum(nnz))
     res, exact = {}, True
     for name, pol, sl in policies:
+        # Team sizes come from the previous run's levels and times
+        # (albba_batch_run): the first run of a batch, with no history, splits
+        # by the instances' shares of the entries; each run on new roots sizes
+        # its teams from other queries; the warm-up and the timed steps repeat
+        # the same roots.
+        b.forget()
+        first = b.run(roots, policy=pol, selector=sl, mapping=mapping)
+        log(f"  {name} first run (no history): {first['total_ms']:.2f} ms")
+        new_ms = []
+        for k in range(steps):
+            o = b.run(other_roots[k], policy=pol, selector=sl, mapping=mapping)
+            new_ms.append(o["total_ms"])
+        log(f"  {name} new roots each run: {', '.join(f'{x:.2f}' for x in new_ms)} ms")
         b.run(roots, policy=pol, selector=sl, mapping=mapping)  # warm-up
-        for i, dg in keep.items():
-            lv, pa = b.results(i)
-            o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
-            exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
         runs = []
         for s in range(steps):
             out = b.run(roots, policy=pol, selector=sl, mapping=mapping)
             runs.append(out)
             log(f"  {name} step {s}: {out['total_ms']:.2f} ms")
+        for i, dg in keep.items():
+            lv, pa = b.results(i)
+            o = dg.bfs(roots[i], policy=A.SWITCH_HEURISTIC)
+            exact &= bool(np.array_equal(lv, o["level"]) and np.array_equal(pa, o["parent"]))
         ms = float(np.mean([o["total_ms"] for o in runs]))
         done = np.concatenate([o["done_ms"] for o in runs])
         serv = np.concatenate([o["ms"] for o in runs])
         by_kind = {k: round(float(np.mean([runs[-1]["ms"][i] for i in range(count) if kinds[i] == k])), 3)
                    for k in sorted(set(kinds))}
         res[name] = {"gteps": round(total / (ms * 1e6), 3), "ms": round(ms, 3),
+                     "first_run_ms": round(first["total_ms"], 3),
+                     "new_roots_ms": round(float(np.mean(new_ms)), 3),
                      "latency_ms": {"p50": round(float(np.percentile(done, 50)), 3),
                                     "p99": round(float(np.percentile(done, 99)), 3),
                                     "service_p50": round(float(np.percentile(serv, 50)), 3),
@@ -462,6 +491,41 @@ def grid_results(ctx, steps=1):
     out = {"n": g.n, "nnz_stored": g.nnz, "roots": len(roots),
            "hm_gteps": round(res["small_frontier_kernel"]["hm_gteps"], 4),
            "detail": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()}}
+    dg.close()
+    return out
+
+
+def rmat_config_results(ctx, scale, nroots, root_seed=3, steps=3):
+    """A plain RMAT configuration on the GPU (ef 16, a/b/c = .57/.19/.19,
+    seeds 1/2, generated on the device, C = 64, sigma = 4096): harmonic-mean
+    GTEPS of the heuristic and the learned switch over the roots (root seed
+    3, as the parity tests), event-timed per BFS and wall clock per step of
+    one albba_bfs_multi call, and whether both policies give the same levels
+    and parents on the first roots."""
+    import albba_amd as A
+
+    dg, g = A.DeviceGraph.generate(ctx, "rmat", 64, 4096, 1, scale=scale, edgefactor=16, a=0.57, b=0.19,
+                                   c=0.19, seed=1, perm_seed=2)
+    roots = g.pick_roots(nroots, root_seed)
+    sel = None
+    if os.path.exists(SELECTOR):
+        sel = A.Selector(ctx)
+        sel.load(SELECTOR)
+    pols = [("heuristic", A.SWITCH_HEURISTIC, None)] + ([("learned", A.SWITCH_LEARNED, sel)] if sel else [])
+    out = {"n": g.n, "nnz_stored": g.nnz, "roots": len(roots)}
+    exact = True
+    for r in roots[:4]:
+        ref = dg.bfs(int(r))
+        for _, pol, sl in pols[1:]:
+            o = dg.bfs(int(r), policy=pol, selector=sl)
+            exact &= bool(np.array_equal(o["level"], ref["level"]) and np.array_equal(o["parent"], ref["parent"]))
+    for name, pol, sl in pols:
+        per_root, step_ms, _ = run_policy(dg, roots, pol, sl, steps)
+        out[name] = {"hm_gteps_events": round(A.harmonic_mean([A.gteps(g.nnz, x) for x in per_root]), 3),
+                     "wall_gteps": round(len(roots) * g.nnz / (step_ms * 1e6), 3)}
+    out["bit_exact"] = exact
+    if sel is not None:
+        sel.close()
     dg.close()
     return out
 
@@ -682,9 +746,14 @@ def main():
     for r in roots[:8]:
         dg.bfs(int(r), policy=pol_main, selector=sel_main, results=True)
     pcie_gteps = min(8, len(roots)) * g.nnz / ((time.perf_counter() - t0) * 1e3 * 1e6)
+    achieved_g = P["bytes_gathered"] / (P["ms"] * 1e-3) if P["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dom,
                 "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic_from_profile(dom),
+                # the same formula over the entries the early-exit scan gathered
+                "achieved_gathered": round(achieved_g / 1e9, 2),
+                "frac_gathered": round(achieved_g / HBM_PEAK, 4),
+                "avg_gathered_bytes_per_launch": round(P["bytes_gathered"] / max(P["launches"], 1)),
                 "avg_launch_ms": round(P["ms"] / max(P["launches"], 1), 5),
                 "avg_bytes_per_launch": round(P["bytes"] / max(P["launches"], 1)),
                 "launches": P["launches"],
@@ -730,12 +799,19 @@ def main():
     dg.close()
     del g
     if not args.no_other_configs:
-        # the high-diameter grid and the batched mix, measured in the same run
+        # the other configurations, measured in the same run: RMAT s16 and s22
+        # on the GPU, the high-diameter grid and the batched mix
+        line["config0_rmat16_gpu"] = rmat_config_results(ctx, 16, 16)
+        log(f"config 0 on the GPU: {line['config0_rmat16_gpu']}")
+        line["config1_rmat22_gpu"] = rmat_config_results(ctx, 22, 64)
+        log(f"config 1 on the GPU: {line['config1_rmat22_gpu']}")
         line["config2_grid4096"] = grid_results(ctx)
         line["config3_batch256"] = batch_results(ctx, 256, 2)
     # a result that is not bit-exact carries no value and fails the run
     ok = (bit_exact is not False and line.get("cpu_baseline", {}).get("levels_match_gpu", True) and
-          line.get("config3_batch256", {}).get("sampled_bit_exact", True))
+          line.get("config3_batch256", {}).get("sampled_bit_exact", True) and
+          line.get("config0_rmat16_gpu", {}).get("bit_exact", True) and
+          line.get("config1_rmat22_gpu", {}).get("bit_exact", True))
     if not ok:
         line["value"] = None
         line["error"] = "levels or parents differ between policies or from the oracle"

@@ -253,6 +253,14 @@ typedef struct albba_bfs_stats {
                                     their gather was skipped (the reference's
                                     width bypass, bfs_par_kernels.c:613,628-631,
                                     counts skipped columns); otherwise 0 (or NULL) */
+    int64_t *swept;              /* [capacity] pull: stored entries of the words
+                                    it scanned, the column entries a full scan
+                                    of those slices reads (the reference's
+                                    SpMV, bfs/src/bfs_par_kernels.c:128-198,
+                                    scans every column of a slice that is not
+                                    bypassed); needs time_levels and a list
+                                    pull (the slice- or column-bypass choice),
+                                    else -1 (or NULL)                             */
 } albba_bfs_stats;
 
 /* Last error message of the calling thread (static storage). */
@@ -437,7 +445,13 @@ ALBBA_API void albba_batch_free(albba_batch *b);
 /* BFS from roots[i] (vertex id) on instance i, all in one launch.  opts: the
  * heuristic (default) or learned policy; mapping[i] (NULL: lane per row)
  * picks instance i's pull mapping; workgroups 0 uses every resident
- * workgroup.  results[count] and total_ms (launch time) may be NULL. */
+ * workgroup.  results[count] and total_ms (launch time) may be NULL.
+ * Team sizes: the first run splits the workgroups by each instance's share of
+ * the stored entries; every later run of the batch sizes them from that
+ * first run's level count and run time of each instance (a high-diameter
+ * instance, whose levels are latency-bound, gets the team that brings it
+ * under the shortest feasible makespan); albba_batch_forget drops that
+ * history. */
 ALBBA_API int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
                               const int32_t *mapping, int32_t workgroups,
                               albba_batch_result *results, float *total_ms);
@@ -454,6 +468,9 @@ ALBBA_API int albba_batch_run(albba_batch *b, const int32_t *roots, const albba_
 ALBBA_API int albba_batch_run_noalloc(albba_batch *b, const int32_t *roots, const albba_bfs_opts *opts,
                                       const int32_t *mapping, int32_t workgroups,
                                       albba_batch_result *results, float *total_ms);
+/* Forget the previous run's per-instance level counts and times: the next
+ * run splits the workgroups by the instances' shares of the stored entries. */
+ALBBA_API void albba_batch_forget(albba_batch *b);
 /* level/parent of instance i after albba_batch_run (either may be NULL) */
 ALBBA_API int albba_batch_results(albba_batch *b, int32_t i, int32_t *level, int32_t *parent);
 

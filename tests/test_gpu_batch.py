@@ -110,3 +110,37 @@ def test_batch_errors(ctx, batch_setup):
         b.run(roots[:3])
     # the batch still works after rejected calls
     check_batch(graphs, inst, roots, b, b.run(roots))
+
+
+def test_batch_team_sizes_from_previous_run(ctx):
+    """From the second run on, teams are sized from the previous run's level
+    counts and times: a long narrow grid (about 2 000 levels, few entries)
+    next to RMAT and ER instances gets a bigger team than its share of the
+    entries gave it, the teams never use more than the workgroups, every
+    run stays exact, and forget() returns to the share split."""
+    graphs = [A.Graph.grid(2000, 4, 0.0, seed=81, perm_seed=82)] + \
+             [A.Graph.rmat(14, 16, seed=83 + 2 * k, perm_seed=84 + 2 * k) for k in range(6)] + \
+             [A.Graph.er(30000, 8.0, seed=97, perm_seed=98)]
+    dgs = [A.DeviceGraph(ctx, g.sellcs(64, 4096, 1)) for g in graphs]
+    roots = np.array([int(g.pick_roots(1, 200 + k, largest_cc=True)[0]) for k, g in enumerate(graphs)],
+                     np.int32)
+    inst = list(range(len(graphs)))
+    b = A.Batch(ctx, dgs)
+    W = 64
+    try:
+        runs = []
+        for k in range(3):
+            if k == 2:
+                b.forget()
+            out = b.run(roots, workgroups=W)
+            check_batch(graphs, inst, roots, b, out)
+            teams = {int(t): int(s) for t, s in zip(out["team"], out["team_size"])}
+            assert sum(teams.values()) <= W
+            runs.append(out)
+        assert runs[0]["num_levels"][0] > 1500
+        assert runs[1]["team_size"][0] > runs[0]["team_size"][0]
+        np.testing.assert_array_equal(runs[2]["team_size"], runs[0]["team_size"])
+    finally:
+        b.close()
+        for dg in dgs:
+            dg.close()

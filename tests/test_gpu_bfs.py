@@ -322,3 +322,30 @@ def test_bfs_multi_matches_single_runs(ctx):
         assert e.value.code == -1
     finally:
         dg.close()
+
+
+def test_swept_entries_of_list_pulls(ctx):
+    """With per-level timing, a list pull reports the stored entries of the
+    words it scanned (what a full scan of those slices reads): at least the
+    entries it gathered, at most every stored entry, never growing from one
+    pull level to the next, and at the root's level every entry but at most
+    the root's own row; push levels report -1."""
+    g = A.Graph.rmat(15, 16, seed=141, perm_seed=142)
+    rp, col = g.row_ptr.copy(), g.col.copy()
+    dg = A.DeviceGraph(ctx, g.sellcs(64, 4096, 1))
+    root = int(g.pick_roots(1, 143)[0])
+    deg = int(rp[root + 1] - rp[root])
+    try:
+        for ch in (A.CHOICE_PULL_SB, A.CHOICE_PULL_CB):
+            o = check(dg, rp, col, root, A.SWITCH_FIXED, ch, time_levels=True)
+            sw, en = o["swept"], o["entries"]
+            assert sw[0] in (g.nnz, g.nnz - deg)
+            assert (sw >= en).all() and (sw <= g.nnz).all()
+            assert (np.diff(sw) <= 0).all()
+        o = check(dg, rp, col, root, A.SWITCH_FIXED, A.CHOICE_PUSH, time_levels=True)
+        assert (o["swept"] == -1).all()
+        # without per-level timing nothing is counted
+        o = dg.bfs(root, policy=A.SWITCH_FIXED, fixed_choice=A.CHOICE_PULL_CB, results=False)
+        assert (o["swept"] == -1).all()
+    finally:
+        dg.close()

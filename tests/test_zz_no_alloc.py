@@ -1,9 +1,11 @@
 """albba_batch_run_noalloc (the batched BFS without any allocation) through
 the C ABI against albba_batch_run: every instance's levels and min-id
 parents bit for bit and the per-instance results equal, on two calls in a
-row (the second on a smaller batch), for both direction policies; batches
-beyond ALBBA_BATCH_NOALLOC_MAX instances are rejected.  The oracle's queue
-BFS checks the no-alloc results as well."""
+row (the second on a smaller batch), for both direction policies, from the
+same team split (each call after albba_batch_forget) and again with the
+teams sized from the previous run; batches beyond ALBBA_BATCH_NOALLOC_MAX
+instances are rejected.  The oracle's queue BFS checks the no-alloc results
+as well."""
 import ctypes as C
 import os
 
@@ -15,13 +17,17 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 NOALLOC_MAX = 1024  # ALBBA_BATCH_NOALLOC_MAX
-FIELDS = ("num_levels", "error", "reached", "edges_in_component", "push_levels", "pull_levels",
-          "team_size", "team")
+RESULT_FIELDS = ("num_levels", "error", "reached", "edges_in_component", "push_levels", "pull_levels")
+FIELDS = RESULT_FIELDS + ("team_size", "team")
 
 
-def run_abi(b, roots, noalloc, policy=A.SWITCH_HEURISTIC, selector=None, mapping=None):
-    """One call of albba_batch_run or albba_batch_run_noalloc through ctypes."""
+def run_abi(b, roots, noalloc, policy=A.SWITCH_HEURISTIC, selector=None, mapping=None, forget=True):
+    """One call of albba_batch_run or albba_batch_run_noalloc through ctypes;
+    with `forget`, from the entries-share team split (the previous run's
+    levels and times dropped first), so that both entry points split alike."""
     L = A.lib()
+    if forget:
+        L.albba_batch_forget(b._p)
     n = len(b.graphs)
     r = np.ascontiguousarray(roots, dtype=np.int32)
     o = A._Opts()
@@ -41,9 +47,9 @@ def run_abi(b, roots, noalloc, policy=A.SWITCH_HEURISTIC, selector=None, mapping
     return out
 
 
-def same(a, b):
+def same(a, b, fields=FIELDS):
     assert a["rc"] == 0 and b["rc"] == 0
-    for k in FIELDS:
+    for k in fields:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
     for i, ((la, pa), (lb, pb)) in enumerate(zip(a["levels"], b["levels"])):
         np.testing.assert_array_equal(la, lb, err_msg=f"instance {i}")
@@ -67,6 +73,9 @@ def test_noalloc_batch_matches_allocating_batch(ctx):
                     ref = run_abi(b, roots, False, pol, s)
                     got = run_abi(b, roots, True, pol, s)
                     same(ref, got)
+                    # teams sized from the run before, still no allocation
+                    hist = run_abi(b, roots, True, pol, s, forget=False)
+                    same(ref, hist, RESULT_FIELDS)
                     for k, gi in enumerate(inst):
                         lv, pa = O.bfs_queue(graphs[gi].row_ptr, graphs[gi].col, roots[k])
                         np.testing.assert_array_equal(got["levels"][k][0], lv)
