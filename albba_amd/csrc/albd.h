@@ -239,7 +239,9 @@ ALBD_HIDDEN int albd_build_row_off(albd_ctx *c, const albd_graph *g, uint16_t *r
 ALBD_HIDDEN int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st,
                               int32_t rp, int32_t ro);
 /* level/parent from position order (st->lp) to vertex order */
-ALBD_HIDDEN int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st);
+/* if_done: queued before the host knew the BFS had ended; the kernel runs only
+ * if the last select kernel found the frontier empty */
+ALBD_HIDDEN int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st, int if_done);
 /* zero st->next (and, flag 1, the per-level slot counters) */
 ALBD_HIDDEN int albd_level_begin(albd_ctx *c, const albd_graph *g, const albd_state *st);
 

@@ -812,8 +812,11 @@ __global__ __launch_bounds__(kQBlock) void k_words_to_queue(albd_graph g, albd_s
 // global access is a coalesced stream.  LDS: 8 bytes per window row.
 constexpr int kMaxWin = 8192;
 
-__global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, albd_state st)
+__global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, albd_state st, int if_done)
 {
+    // queued before the host knew the BFS had ended: run only if the last
+    // select kernel found the frontier empty
+    if (if_done && st.ctr->frontier != 0) return;
     extern __shared__ int32_t s_win[];
     int32_t *s_level = s_win, *s_parent = s_win + g.win;
     constexpr int kV = 4; // positions in flight per thread
@@ -867,8 +870,9 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
 // consecutive vertex ids and gathers lp through pos_of.
 constexpr int kVertexChunk = 4096;
 
-__global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st)
+__global__ __launch_bounds__(kBlock) void k_to_vertex_order(albd_graph g, albd_state st, int if_done)
 {
+    if (if_done && st.ctr->frontier != 0) return;
     constexpr int kV = 4; // vertices in flight per thread
     for (int64_t c0 = (int64_t)blockIdx.x * kVertexChunk; c0 < g.n; c0 += (int64_t)gridDim.x * kVertexChunk) {
         for (int64_t v0 = c0 + threadIdx.x; v0 < c0 + kVertexChunk && v0 < g.n; v0 += kBlock * kV) {
@@ -1396,21 +1400,21 @@ int albd_bfs_init(albd_ctx *c, const albd_graph *g, const albd_state *st, int32_
     return 0;
 }
 
-int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st)
+int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st, int if_done)
 {
     if (g->win >= 1 && g->win <= kMaxWin && g->row_off) {
         int64_t blocks = ((int64_t)g->n + g->win - 1) / g->win;
         int64_t cap = (int64_t)albd_num_cus(c) * 16;
         hipLaunchKernelGGL(k_to_vertex_order_win,
                            dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))), dim3(kBlock),
-                           sizeof(int32_t) * 2 * (size_t)g->win, albd_stream(c), *g, *st);
+                           sizeof(int32_t) * 2 * (size_t)g->win, albd_stream(c), *g, *st, if_done);
         HIPCHK(hipGetLastError());
         return 0;
     }
     int64_t blocks = ((int64_t)g->n + kVertexChunk - 1) / kVertexChunk;
     int64_t cap = (int64_t)albd_num_cus(c) * 32;
     hipLaunchKernelGGL(k_to_vertex_order, dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))),
-                       dim3(kBlock), 0, albd_stream(c), *g, *st);
+                       dim3(kBlock), 0, albd_stream(c), *g, *st, if_done);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -1097,6 +1097,11 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
     /* kernels already queued for the coming level: the choice whose pull is
      * queued, or -1; whether the push kernels are */
     int queued_pull = -1, queued_push = 0;
+    /* the vertex-order pass queued behind the last posted level (it runs only
+     * if that level's select kernel finds the frontier empty), and the
+     * frontier of the level before the current one */
+    int vo_queued = 0;
+    int64_t frontier_prev = 0;
     /* init, then the select kernel turns the root into level 0's frontier */
     if ((rc = albd_event_record(d, ev.start)) || (rc = albd_bfs_init(d, &G->g, &G->st, rp, root)))
         return albba_fail(rc, "bfs: %s", albd_error());
@@ -1254,6 +1259,18 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
                     return albba_fail(rc, "bfs: %s", albd_error());
                 queued_pull = guess;
             }
+            /* A power-law BFS ends with a level of a few vertices after one of
+             * thousands (RMAT s24: 1-141 after 1 222-42 840): after such a
+             * level the vertex-order pass goes behind the next level's
+             * kernels too, so that the device does not wait for the host to
+             * read that the frontier is empty.  A wrong guess costs one
+             * kernel that returns at once. */
+            vo_queued = 0;
+            if (!opts.time_levels && !small && nlev == 1 && frontier <= 1024 && 64 * frontier < frontier_prev) {
+                if ((rc = albd_to_vertex_order(d, &G->g, &G->st, 1)))
+                    return albba_fail(rc, "bfs: %s", albd_error());
+                vo_queued = 1;
+            }
             if ((rc = albd_mail_wait(d, &G->st, seq))) return albba_fail(rc, "bfs: %s", albd_error());
         } else if ((rc = albd_select_read(d, &G->g, &G->st, &sel))) {
             return albba_fail(rc, "bfs: %s", albd_error());
@@ -1298,11 +1315,12 @@ static int bfs_run(albba_ctx *ctx, albba_graph *G, int32_t root, const albba_bfs
         }
         if (!spec) commit_pointers(G, &ls, choice, 1);
         commit_counts(&ls, choice, &sums);
+        frontier_prev = frontier;
         frontier = (int64_t)hc->frontier;
         level += nlev;
     }
     float ms = 0;
-    if ((rc = albd_to_vertex_order(d, &G->g, &G->st)) || (rc = albd_event_record(d, ev.end)) ||
+    if ((!vo_queued && (rc = albd_to_vertex_order(d, &G->g, &G->st, 0))) || (rc = albd_event_record(d, ev.end)) ||
         (!ev.defer && (rc = albd_event_elapsed(d, ev.start, ev.end, &ms))))
         return albba_fail(rc, "bfs: %s", albd_error());
     if (stats) {
