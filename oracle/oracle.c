@@ -216,9 +216,28 @@ int orc_albba_bfs(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level
                                 capacity, INT32_MAX);
 }
 
+static int albba_loop(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                      int32_t *frontier_out, int32_t *mode_out, int64_t *bys_out, int64_t *byc_out,
+                      int32_t capacity, int32_t max_levels, int f9);
+
 int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
                          int32_t *frontier_out, int32_t *mode_out, int64_t *bys_out, int64_t *byc_out,
                          int32_t capacity, int32_t max_levels)
+{
+    return albba_loop(s, root, tasks, level, parent, frontier_out, mode_out, bys_out, byc_out, capacity,
+                      max_levels, 0);
+}
+
+int orc_albba_bfs_f9(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                     int32_t *frontier_out, int32_t *mode_out, int32_t capacity)
+{
+    return albba_loop(s, root, tasks, level, parent, frontier_out, mode_out, NULL, NULL, capacity, INT32_MAX,
+                      1);
+}
+
+static int albba_loop(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                      int32_t *frontier_out, int32_t *mode_out, int64_t *bys_out, int64_t *byc_out,
+                      int32_t capacity, int32_t max_levels, int f9)
 {
     const int32_t n = s->n, C = s->C, ns = s->nslices;
     if (root < 0 || root >= n || tasks < 1) return -1;
@@ -240,6 +259,9 @@ int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t
         !smin || !tp || !tcnt || !tbs || !tbc)
         return -2;
     for (int32_t p = 0; p < n; ++p) pos_of[s->row_order[p]] = p;
+    /* entries the F9 offsets leave unwritten hold a visited vertex (the
+     * root): a stale, harmless frontier entry rather than undefined memory */
+    for (int32_t i = 0; i < n; ++i) colidx_x[i] = root;
     for (int32_t sl = 0; sl < ns; ++sl) { /* bfs_SELLCSv2.c:170-182 */
         smin[sl] = n;
         for (int32_t v = s->vop_pointers[sl]; v < s->vop_pointers[sl + 1]; ++v) {
@@ -324,6 +346,24 @@ int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t
         if (pull) {
             if (nnzy > T) {
                 memcpy(mask_x, mask_y, sizeof(int32_t) * (size_t)n); /* spmv_merge_res */
+            } else if (f9) {
+                /* the reference's dense2sparse (bfs_SELLCSv2.c:488-499,
+                 * bfs_par_kernels.c:754-787): task t scans the ids
+                 * [tp[t] << 8, tp[t + 1] << 8) -- its slices' positions when
+                 * C = 256 -- and writes them from the prefix of the tasks'
+                 * counts, which count its slices' rows by position.  Where a
+                 * sigma window crosses a task boundary the two differ: a task
+                 * overwrites the next one's first entries (tasks run in order
+                 * here) and leaves stale entries in its own range. */
+                int32_t off = 0;
+                for (int32_t t = 0; t < tasks; ++t) {
+                    const int64_t lo = (int64_t)tp[t] << 8;
+                    const int64_t hi = ((int64_t)tp[t + 1] << 8) < n ? ((int64_t)tp[t + 1] << 8) : n;
+                    int32_t k = off;
+                    for (int64_t v = lo; v < hi; ++v)
+                        if (mask_y[v] > 0 && k < n) colidx_x[k++] = (int32_t)v;
+                    off += tcnt[t];
+                }
             } else {
                 /* dense2sparse over ids, offsets from counts of the same ids */
                 int32_t k = 0;

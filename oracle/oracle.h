@@ -72,6 +72,12 @@ int orc_albba_bfs_levels(const orc_sell *s, int32_t root, int32_t tasks, int32_t
                          int32_t *frontier, int32_t *mode, int64_t *bypass_slice, int64_t *bypass_col,
                          int32_t capacity, int32_t max_levels);
 
+/* orc_albba_bfs with the reference's dense2sparse offsets (the F9 defect)
+ * in place of the corrected ones, to show where it loses frontier vertices;
+ * meaningful at C = 256, the slice width the reference's `<< 8` assumes. */
+int orc_albba_bfs_f9(const orc_sell *s, int32_t root, int32_t tasks, int32_t *level, int32_t *parent,
+                     int32_t *frontier, int32_t *mode, int32_t capacity);
+
 /* Graph500-style BFS tree check on W: 0 if valid, else a negative code
  *  -1 root wrong, -2 parent of a reached vertex unreached or level gap != 1,
  *  -3 tree edge missing from the graph, -4 an edge skips a level,

@@ -56,6 +56,9 @@ def orc() -> C.CDLL:
         L.orc_albba_bfs_levels.argtypes = [C.POINTER(OrcSell), C.c_int32, C.c_int32, I32P, I32P, I32P,
                                            I32P, I64P, I64P, C.c_int32, C.c_int32]
         L.orc_albba_bfs_levels.restype = C.c_int
+        L.orc_albba_bfs_f9.argtypes = [C.POINTER(OrcSell), C.c_int32, C.c_int32, I32P, I32P, I32P, I32P,
+                                       C.c_int32]
+        L.orc_albba_bfs_f9.restype = C.c_int
         L.orc_validate.argtypes = [C.c_int32, I64P, I32P, C.c_int32, I32P, I32P]
         L.orc_validate.restype = C.c_int
         _orc = L
@@ -179,6 +182,21 @@ def albba_bfs_on(s: OrcSell, n: int, root: int, tasks: int = 8, capacity: int = 
     m = min(k, capacity)
     return {"level": lv, "parent": pa, "num_levels": k, "frontier": fr[:m].copy(),
             "mode": md[:m].copy(), "bypass_slice": bs[:m].copy(), "bypass_col": bc[:m].copy()}
+
+
+def albba_bfs_f9(s: OrcSell, n: int, root: int, tasks: int = 8, capacity: int = 1 << 16):
+    """The level loop with the reference's dense2sparse offsets (F9): shows
+    the defect, never used as a reference result."""
+    lv = np.empty(n, np.int32)
+    pa = np.empty(n, np.int32)
+    fr = np.zeros(capacity, np.int32)
+    md = np.zeros(capacity, np.int32)
+    k = orc().orc_albba_bfs_f9(C.byref(s), root, tasks, lv.ctypes.data_as(I32P), pa.ctypes.data_as(I32P),
+                               fr.ctypes.data_as(I32P), md.ctypes.data_as(I32P), capacity)
+    if k < 0:
+        raise RuntimeError(f"orc_albba_bfs_f9 failed: {k}")
+    m = min(k, capacity)
+    return {"level": lv, "parent": pa, "num_levels": k, "frontier": fr[:m].copy(), "mode": md[:m].copy()}
 
 
 def build_sell(row_ptr, col, C_: int, sigma: int):

@@ -147,6 +147,42 @@ def test_albba_restatement_covers_every_conversion():
     assert seen == {(0, 0), (0, 1), (1, 0), (1, 1)}
 
 
+def test_f9_defect_loses_vertices_only_where_windows_cross_tasks():
+    """The reference's dense2sparse (bfs/src/bfs_SELLCSv2.c:488-499,
+    bfs/src/bfs_par_kernels.c:754-787) scans task t's id range but writes from
+    offsets counted over task t's slice rows.  With sigma = C = 256 every
+    window is one slice, the two agree and the level loop is exact; with
+    sigma = 16384 windows cross task boundaries, frontier vertices are
+    overwritten at a pull -> push switch, and the vertices only they reach
+    come later or never.  The corrected restatement is exact on both."""
+    import albba_amd as A
+
+    g = A.Graph.rmat(14, 16, seed=224, perm_seed=3)
+    rp, col = g.row_ptr.copy(), g.col.copy()
+    n = rp.size - 1
+    roots = [int(r) for r in g.pick_roots(16, 9)]
+    for sigma in (256, 16384):
+        s = O.build_sell(rp, col, 256, sigma)
+        try:
+            wrong = 0
+            for root in roots:
+                lv, pa = O.bfs_queue(rp, col, root)
+                fixed = O.albba_bfs_on(s, n, root, 8)
+                np.testing.assert_array_equal(fixed["level"], lv)
+                np.testing.assert_array_equal(fixed["parent"], pa)
+                r = O.albba_bfs_f9(s, n, root, 8)
+                bad = r["level"] != lv
+                # lost frontier vertices delay what they reach, never hasten it
+                assert np.all((r["level"][bad] == -1) | (r["level"][bad] > lv[bad]))
+                wrong += int(bad.sum())
+            if sigma == 256:
+                assert wrong == 0
+            else:
+                assert wrong > 0
+        finally:
+            O.free_sell(s)
+
+
 def test_min_id_parent_closed_form():
     rng = np.random.default_rng(5)
     for trial in range(20):
