@@ -251,6 +251,11 @@ struct RowArgs {
     const int64_t *rm_ptr;  // row-major copy: pointers over positions
     int32_t *rm_col;
     unsigned int *push_cnt; // directed: out-degree histogram over positions, or null
+    // 64-row groups holding more than kHeavyGroup entries are listed here
+    // (heavy_stat[0]: their count, [1]: the most entries of one) and copied
+    // by k_rowmajor_heavy
+    int32_t *heavy_list;
+    unsigned long long *heavy_stat;
     int32_t n;
 };
 
@@ -267,62 +272,110 @@ __device__ __forceinline__ void copy_entry(const RowArgs &a, int64_t src, int64_
 }
 
 // 6a. the row-major copy (columns as positions).  A wave takes 64
-// consecutive positions and spreads their entries evenly over its lanes, so
-// that neighbouring lanes read neighbouring entries of one row (a lane
-// walking a row of its own would have every lane of every resident wave on a
-// different line).  Four steps of 64 entries are handled together so that
-// their position gathers are in flight at once; the streamed reads of the
-// CSR columns and the stores of the copy are non-temporal, which leaves the
-// caches to the gathered position table.  Rows longer than kLongRow are left
-// to k_rowmajor_long.
-__global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
+// consecutive positions (a group) and spreads their entries evenly over its
+// lanes, so that neighbouring lanes read neighbouring entries of one row (a
+// lane walking a row of its own would have every lane of every resident wave
+// on a different line).  Four steps of 64 entries are handled together so
+// that their position gathers are in flight at once; the streamed reads of
+// the CSR columns and the stores of the copy are non-temporal, which leaves
+// the caches to the gathered position table.  Rows longer than kLongRow are
+// left to k_rowmajor_long.  Groups are dealt to waves by position, and the
+// first group of every sigma window holds its hubs, so a few waves would hold
+// most of the entries (RMAT s24: the busiest wave 41 times the mean); groups
+// of more than kHeavyGroup entries are listed instead and copied in chunks of
+// kHeavyChunk entries by k_rowmajor_heavy.
+constexpr unsigned long long kHeavyGroup = 8192;
+constexpr int kHeavyChunk = 4096; // per block: kHeavyChunk / 4 per wave
+
+// entries [e_lo, e_hi) of a group: ex is the lane rows' exclusive prefix of
+// their lengths, start and rm the lane row's first CSR and row-major entry
+__device__ __forceinline__ void copy_group(const RowArgs &a, int lane, unsigned long long ex,
+                                           unsigned long long e_lo, unsigned long long e_hi, int64_t start,
+                                           int64_t rm)
 {
     constexpr int U = 4;
+    for (unsigned long long e0 = e_lo; e0 < e_hi; e0 += 64 * U) {
+        int64_t src[U], dst[U];
+        bool act[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned long long e = e0 + (unsigned long long)(u * 64 + lane);
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const unsigned long long x = __shfl(ex, lo + step, 64);
+                if (x <= e) lo += step;
+            }
+            const int64_t j = (int64_t)(e - __shfl(ex, lo, 64));
+            src[u] = __shfl(start, lo, 64) + j;
+            dst[u] = __shfl(rm, lo, 64) + j;
+            act[u] = e < e_hi;
+        }
+        int32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = act[u] ? __builtin_nontemporal_load(&a.col[src[u]]) : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = act[u] ? position_of(a, c[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!act[u]) continue;
+            __builtin_nontemporal_store(c[u], &a.rm_col[dst[u]]);
+            if (a.push_cnt) atomicAdd(&a.push_cnt[c[u]], 1u);
+        }
+    }
+}
+
+// the group at positions p0 .. p0 + 63 as seen by one wave: its lane's row
+// (rows longer than kLongRow count as empty), the prefix and the total
+struct Group {
+    int64_t start, rm;
+    unsigned long long ex, tot;
+};
+
+__device__ __forceinline__ Group load_group(const RowArgs &a, int64_t p0, int lane)
+{
+    Group gr{0, 0, 0, 0};
+    const int64_t p = p0 + lane;
+    int32_t l = 0;
+    if (p < a.n) {
+        l = a.row_len[p];
+        if (l > kLongRow) l = 0;
+        gr.start = a.rp[a.row_order[p]];
+        gr.rm = a.rm_ptr[p];
+    }
+    gr.ex = wave_exclusive_scan<unsigned long long>((unsigned long long)l, &gr.tot);
+    return gr;
+}
+
+__global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
+{
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
     for (int64_t p0 = wave * 64; p0 < a.n; p0 += nwaves * 64) {
-        const int64_t p = p0 + lane;
-        int64_t start = 0, rm = 0;
-        int32_t l = 0;
-        if (p < a.n) {
-            l = a.row_len[p];
-            if (l > kLongRow) l = 0;
-            start = a.rp[a.row_order[p]];
-            rm = a.rm_ptr[p];
-        }
-        unsigned long long tot;
-        const unsigned long long ex = wave_exclusive_scan<unsigned long long>((unsigned long long)l, &tot);
-        for (unsigned long long e0 = 0; e0 < tot; e0 += 64 * U) {
-            int64_t src[U], dst[U];
-            bool act[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned long long e = e0 + (unsigned long long)(u * 64 + lane);
-                int lo = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    const unsigned long long x = __shfl(ex, lo + step, 64);
-                    if (x <= e) lo += step;
-                }
-                const int64_t j = (int64_t)(e - __shfl(ex, lo, 64));
-                src[u] = __shfl(start, lo, 64) + j;
-                dst[u] = __shfl(rm, lo, 64) + j;
-                act[u] = e < tot;
+        const Group gr = load_group(a, p0, lane);
+        if (gr.tot > kHeavyGroup) {
+            if (lane == 0) {
+                a.heavy_list[atomicAdd(&a.heavy_stat[0], 1ull)] = (int32_t)(p0 >> 6);
+                atomicMax(&a.heavy_stat[1], gr.tot);
             }
-            int32_t c[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) c[u] = act[u] ? __builtin_nontemporal_load(&a.col[src[u]]) : 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) c[u] = act[u] ? position_of(a, c[u]) : 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (!act[u]) continue;
-                __builtin_nontemporal_store(c[u], &a.rm_col[dst[u]]);
-                if (a.push_cnt) atomicAdd(&a.push_cnt[c[u]], 1u);
-            }
+            continue;
         }
+        copy_group(a, lane, gr.ex, 0, gr.tot, gr.start, gr.rm);
     }
+}
+
+// 6a''. the listed groups: block (x, y) copies entries [x * kHeavyChunk,
+// (x + 1) * kHeavyChunk) of group list[y], a quarter per wave
+__global__ __launch_bounds__(kBB) void k_rowmajor_heavy(RowArgs a, const int32_t *list)
+{
+    constexpr int kPerWave = kHeavyChunk / (kBB / 64);
+    const int lane = lane_id();
+    const Group gr = load_group(a, (int64_t)list[blockIdx.y] * 64, lane);
+    const unsigned long long lo = (unsigned long long)blockIdx.x * kHeavyChunk + (threadIdx.x >> 6) * kPerWave;
+    if (lo >= gr.tot) return;
+    const unsigned long long hi = lo + kPerWave < gr.tot ? lo + kPerWave : gr.tot;
+    copy_group(a, lane, gr.ex, lo, hi, gr.start, gr.rm);
 }
 
 // the in-window offsets of the vertices' positions (power-of-two windows)
@@ -745,8 +798,23 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     ra.rm_col = rm_col;
     ra.push_cnt = cnt;
     ra.n = n;
+    const int64_t ngroups = ((int64_t)n + 63) / 64;
+    if ((rc = T.get(ngroups > 0 ? ngroups : 1, &ra.heavy_list)) || (rc = T.get(2, &ra.heavy_stat))) return rc;
+    HIPCHK(hipMemsetAsync(ra.heavy_stat, 0, 2 * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_rowmajor, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, ra);
     HIPCHK(hipGetLastError());
+    {
+        unsigned long long hs[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(hs, ra.heavy_stat, sizeof hs, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const unsigned int chunks = (unsigned int)((hs[1] + kHeavyChunk - 1) / kHeavyChunk);
+        // grid y is limited to 65535: take the groups in batches
+        for (unsigned long long y0 = 0; y0 < hs[0]; y0 += 65535ull) {
+            const unsigned int ny = (unsigned int)(hs[0] - y0 < 65535ull ? hs[0] - y0 : 65535ull);
+            hipLaunchKernelGGL(k_rowmajor_heavy, dim3(chunks, ny), dim3(kBB), 0, s, ra, ra.heavy_list + y0);
+            HIPCHK(hipGetLastError());
+        }
+    }
     if (maxlen > kLongRow) {
         int32_t *list = nullptr;
         unsigned int *count = nullptr;
