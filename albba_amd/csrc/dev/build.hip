@@ -279,12 +279,14 @@ __device__ __forceinline__ void copy_entry(const RowArgs &a, int64_t src, int64_
 // that their position gathers are in flight at once; the streamed reads of
 // the CSR columns and the stores of the copy are non-temporal, which leaves
 // the caches to the gathered position table.  Rows longer than kLongRow are
-// left to k_rowmajor_long.  Groups are dealt to waves by position, and the
-// first group of every sigma window holds its hubs, so a few waves would hold
-// most of the entries (RMAT s24: the busiest wave 41 times the mean); groups
-// of more than kHeavyGroup entries are listed instead and copied in chunks of
-// kHeavyChunk entries by k_rowmajor_heavy.
-constexpr unsigned long long kHeavyGroup = 8192;
+// left to k_rowmajor_long.  The first groups of every sigma window hold its
+// hubs, so groups dealt to waves by position would leave a few waves most of
+// the entries (RMAT s24, waves striding by their count: the busiest wave 41
+// times the mean); groups of more than kHeavyGroup entries are listed instead
+// and copied in chunks of kHeavyChunk entries by k_rowmajor_heavy, and each
+// wave takes a run of consecutive groups, which spreads a window's heavier
+// groups over several waves.
+constexpr unsigned long long kHeavyGroup = 2048;
 constexpr int kHeavyChunk = 4096; // per block: kHeavyChunk / 4 per wave
 
 // entries [e_lo, e_hi) of a group: ex is the lane rows' exclusive prefix of
@@ -352,7 +354,11 @@ __global__ __launch_bounds__(kBB) void k_rowmajor(RowArgs a)
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
-    for (int64_t p0 = wave * 64; p0 < a.n; p0 += nwaves * 64) {
+    const int64_t ngroups = ((int64_t)a.n + 63) / 64;
+    const int64_t per = (ngroups + nwaves - 1) / nwaves;
+    const int64_t g1 = (wave + 1) * per < ngroups ? (wave + 1) * per : ngroups;
+    for (int64_t g = wave * per; g < g1; ++g) {
+        const int64_t p0 = g * 64;
         const Group gr = load_group(a, p0, lane);
         if (gr.tot > kHeavyGroup) {
             if (lane == 0) {
