@@ -464,6 +464,75 @@ __global__ __launch_bounds__(kBB) void k_colmajor(const int64_t *rm_ptr, const i
     }
 }
 
+// 6b'. C = 64: the same columns, 16 at a time through LDS.  Load k of a
+// 16-column step reads rows 4k .. 4k + 3, 16 consecutive entries of each
+// (lane l: row 4k + l / 16, column l % 16 of the step), so that one load
+// touches a few lines where one per row (lanes as rows) touches 64; the
+// values go to a 16 x 65 tile (the odd stride spreads a load's 16-lane row
+// groups over the banks) and come back one column per store, lanes as rows.
+// Lanes past a column's active rows read entries they do not store (clamped
+// to the copy).
+constexpr int kColStep = 16;
+
+__global__ __launch_bounds__(kBB) void k_colmajor64(const int64_t *rm_ptr, const int32_t *rm_col,
+                                                    const int32_t *slice_of, const int64_t *vop_ptr,
+                                                    const uint8_t *vop_len, const int64_t *cstart, int64_t nvops,
+                                                    int64_t rm_len, int32_t *colidx)
+{
+    __shared__ int32_t tile[kBB / 64][kColStep * 65];
+    int32_t *t = tile[threadIdx.x >> 6];
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * kBB) >> 6;
+    const int64_t last = rm_len > 0 ? rm_len - 1 : 0;
+    const int vl = lane & (kColStep - 1), rsub = lane >> 4;
+    for (int64_t q0 = wave * 64; q0 < nvops; q0 += nwaves * 64) {
+        const int64_t q = q0 + lane;
+        int32_t sl = 0, act = 0;
+        int64_t j = 0, base = 0;
+        if (q < nvops) {
+            sl = slice_of[q];
+            j = q - vop_ptr[sl];
+            act = (int32_t)vop_len[q] + 1;
+            base = cstart[q];
+        }
+        const int cnt = (int)min((int64_t)64, nvops - q0);
+        for (int i0 = 0; i0 < cnt; i0 += kColStep) {
+            // this lane's column of the step (past the wave's last: that one
+            // again, not stored)
+            const int v = i0 + vl < cnt ? i0 + vl : cnt - 1;
+            const int64_t s_v = __shfl(sl, v, 64);
+            const int64_t j_v = __shfl(j, v, 64);
+            int64_t rp[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) rp[k] = rm_ptr[s_v * 64 + 4 * k + rsub];
+            int32_t val[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int64_t e = rp[k] + j_v;
+                val[k] = rm_col[e < last ? e : last];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) t[vl * 65 + 4 * k + rsub] = val[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int k = 0; k < kColStep; ++k) {
+                const int i = i0 + k;
+                if (i >= cnt) break;
+                const int64_t b_i = readlane64(base, i);
+                const int32_t a_i = __builtin_amdgcn_readlane(act, i);
+                if (lane < a_i) colidx[b_i + lane] = t[k * 65 + lane];
+            }
+            // the next step rewrites the tile
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBB) void k_widen_u32(const unsigned int *in, int64_t count, int64_t *out)
 {
     for (int64_t i = (int64_t)blockIdx.x * kBB + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBB)
@@ -840,8 +909,12 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         }
     }
     if (nvops > 0) {
-        hipLaunchKernelGGL(k_colmajor, dim3(grid_for(nvops, kBB)), dim3(kBB), 0, s, rm_ptr, rm_col, slice_of,
-                           vop_ptr, vop_len, cstart, nvops, C, colidx);
+        if (C == 64)
+            hipLaunchKernelGGL(k_colmajor64, dim3(grid_for(nvops, kBB)), dim3(kBB), 0, s, rm_ptr, rm_col, slice_of,
+                               vop_ptr, vop_len, cstart, nvops, nnz, colidx);
+        else
+            hipLaunchKernelGGL(k_colmajor, dim3(grid_for(nvops, kBB)), dim3(kBB), 0, s, rm_ptr, rm_col, slice_of,
+                               vop_ptr, vop_len, cstart, nvops, C, colidx);
         HIPCHK(hipGetLastError());
     }
 
