@@ -20,8 +20,8 @@
 //      CSR when the matrix is symmetric, the pull CSR otherwise), written
 //      coalesced, then the column-major layout from it: lanes are a slice's
 //      rows, so each column is one coalesced store;
-//   7. directed graphs: the push CSR (transpose over positions) by a degree
-//      histogram, a scan, an atomic fill and a per-row sort.
+//   7. directed graphs: the push CSR (transpose over positions) by one radix
+//      sort of (column, row) keys.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
@@ -250,7 +250,6 @@ struct RowArgs {
     int32_t win_mask;
     const int64_t *rm_ptr;  // row-major copy: pointers over positions
     int32_t *rm_col;
-    unsigned int *push_cnt; // directed: out-degree histogram over positions, or null
     // 64-row groups holding more than kHeavyGroup entries are listed here
     // (heavy_stat[0]: their count, [1]: the most entries of one) and copied
     // by k_rowmajor_heavy
@@ -268,7 +267,6 @@ __device__ __forceinline__ void copy_entry(const RowArgs &a, int64_t src, int64_
 {
     const int32_t c = position_of(a, a.col[src]);
     a.rm_col[dst] = c;
-    if (a.push_cnt) atomicAdd(&a.push_cnt[c], 1u);
 }
 
 // 6a. the row-major copy (columns as positions).  A wave takes 64
@@ -322,7 +320,6 @@ __device__ __forceinline__ void copy_group(const RowArgs &a, int lane, unsigned 
         for (int u = 0; u < U; ++u) {
             if (!act[u]) continue;
             __builtin_nontemporal_store(c[u], &a.rm_col[dst[u]]);
-            if (a.push_cnt) atomicAdd(&a.push_cnt[c[u]], 1u);
         }
     }
 }
@@ -539,11 +536,15 @@ __global__ __launch_bounds__(kBB) void k_widen_u32(const unsigned int *in, int64
         out[i] = in[i];
 }
 
-// 7. directed: the transpose over positions; entries land in any order and
-// are sorted per row afterwards
-__global__ __launch_bounds__(kBB) void k_push_fill(const int64_t *rm_ptr, const int32_t *rm_col, int32_t n,
-                                                   const int64_t *push_ptr, unsigned int *cursor,
-                                                   int32_t *push_col)
+// 7. a transpose by one sort (the directed push CSR over positions, and
+// W = A^T of a loaded file): entry (r, c) becomes the key c << shift | r, a
+// radix sort orders the keys by column and, within a column, by row, and one
+// pass splits them into the transpose's columns and row pointers.  Atomic
+// placement into per-column cursors serialised on the hubs' columns (RMAT
+// s22: 20 ms for the fill of one directed build) and still needed a sort of
+// every row.
+__global__ __launch_bounds__(kBB) void k_transpose_keys(const int64_t *rp, const int32_t *col, int32_t n,
+                                                        int shift, unsigned long long *keys)
 {
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kBB + threadIdx.x) >> 6;
@@ -553,17 +554,32 @@ __global__ __launch_bounds__(kBB) void k_push_fill(const int64_t *rm_ptr, const 
         int64_t start = 0;
         int32_t l = 0;
         if (p < n) {
-            start = rm_ptr[p];
-            l = (int32_t)(rm_ptr[p + 1] - start);
+            start = rp[p];
+            l = (int32_t)(rp[p + 1] - start);
         }
         wave_rows_foreach(l, [&](int i, int64_t j, bool act) {
             const int64_t s = __shfl(start, i, 64);
-            if (act) {
-                const int32_t q = rm_col[s + j];
-                const unsigned int at = atomicAdd(&cursor[q], 1u);
-                push_col[push_ptr[q] + at] = (int32_t)(p0 + i);
-            }
+            if (act)
+                keys[s + j] = ((unsigned long long)(uint32_t)col[s + j] << shift) | (unsigned long long)(p0 + i);
         });
+    }
+}
+
+// the sorted keys' rows are the transpose's entries; the first key of each
+// column sets the row pointers of that column and of the empty columns
+// before it (the last key those after it)
+__global__ __launch_bounds__(kBB) void k_transpose_split(const unsigned long long *sorted, int64_t nnz, int shift,
+                                                         int32_t ncols, int64_t *tp, int32_t *tc)
+{
+    const unsigned long long mask = (1ull << shift) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * kBB + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * kBB) {
+        const unsigned long long k = sorted[i];
+        tc[i] = (int32_t)(k & mask);
+        const int64_t cc = (int64_t)(k >> shift);
+        const int64_t cp = i == 0 ? -1 : (int64_t)(sorted[i - 1] >> shift);
+        for (int64_t x = cp + 1; x <= cc; ++x) tp[x] = i;
+        if (i == nnz - 1)
+            for (int64_t x = cc + 1; x <= ncols; ++x) tp[x] = nnz;
     }
 }
 
@@ -581,12 +597,6 @@ __global__ __launch_bounds__(kBB) void k_widen_rp(const int32_t *rp32, int32_t n
         if (i < n && rp32[i + 1] < v) bad = true;
     }
     if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(&flags[0], 1ull);
-}
-
-__global__ __launch_bounds__(kBB) void k_hist(const int32_t *col, int64_t nnz, unsigned int *cnt)
-{
-    for (int64_t k = (int64_t)blockIdx.x * kBB + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * kBB)
-        atomicAdd(&cnt[col[k]], 1u);
 }
 
 int bits_for(uint64_t x)
@@ -635,6 +645,35 @@ int scan_i64(albd_ctx *c, Temps &T, const int64_t *in, int64_t *out, size_t coun
     if (rc) return rc;
     ROCCHK(rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, count, rocprim::plus<int64_t>(),
                                    albd_stream(c)));
+    return 0;
+}
+
+// the transpose of an nrows x ncols CSR (rp, col) into (tp[ncols + 1], tc),
+// each transposed row listing its entries' rows in ascending order
+int transpose_by_sort(albd_ctx *c, Temps &T, const int64_t *rp, const int32_t *col, int32_t nrows, int32_t ncols,
+                      int64_t nnz, int64_t *tp, int32_t *tc)
+{
+    hipStream_t s = albd_stream(c);
+    if (nnz <= 0) {
+        HIPCHK(hipMemsetAsync(tp, 0, sizeof(int64_t) * ((size_t)ncols + 1), s));
+        return 0;
+    }
+    const int shift = bits_for((uint64_t)(nrows > 1 ? nrows - 1 : 0));
+    int end_bit = shift + bits_for((uint64_t)(ncols > 1 ? ncols - 1 : 0));
+    if (end_bit < 1) end_bit = 1;
+    unsigned long long *keys = nullptr, *sorted = nullptr;
+    int rc;
+    if ((rc = T.get(nnz, &keys)) || (rc = T.get(nnz, &sorted))) return rc;
+    hipLaunchKernelGGL(k_transpose_keys, dim3(grid_for(nrows, kBB)), dim3(kBB), 0, s, rp, col, nrows, shift, keys);
+    HIPCHK(hipGetLastError());
+    size_t bytes = 0;
+    ROCCHK(rocprim::radix_sort_keys(nullptr, bytes, keys, sorted, (size_t)nnz, 0, end_bit, s));
+    unsigned char *tmp = nullptr;
+    if ((rc = T.get(bytes ? bytes : 1, &tmp))) return rc;
+    ROCCHK(rocprim::radix_sort_keys(tmp, bytes, keys, sorted, (size_t)nnz, 0, end_bit, s));
+    hipLaunchKernelGGL(k_transpose_split, dim3(grid_for(nnz, kBB)), dim3(kBB), 0, s, sorted, nnz, shift, ncols, tp,
+                       tc);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -848,11 +887,6 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         HIPCHK(hipMemsetAsync(len64 + npos, 0, sizeof(int64_t), s));
         if ((rc = scan_i64(c, T, len64, rm_ptr, (size_t)(npos + 1)))) return rc;
     }
-    unsigned int *cnt = nullptr;
-    if (in->symmetric != 1) {
-        if ((rc = T.get(npos + 1, &cnt))) return rc;
-        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(npos + 1), s));
-    }
     RowArgs ra;
     ra.rp = rp;
     ra.col = col;
@@ -871,7 +905,6 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
     }
     ra.rm_ptr = rm_ptr;
     ra.rm_col = rm_col;
-    ra.push_cnt = cnt;
     ra.n = n;
     const int64_t ngroups = ((int64_t)n + 63) / 64;
     if ((rc = T.get(ngroups > 0 ? ngroups : 1, &ra.heavy_list)) || (rc = T.get(2, &ra.heavy_stat))) return rc;
@@ -925,24 +958,7 @@ extern "C" int albd_build_graph(albd_ctx *c, const albd_build_input *in, albd_gr
         g->push_ptr = push_ptr;
         g->push_col = push_col;
         if (!push_ptr || !push_col) return ALBBA_EDEVMEM;
-        hipLaunchKernelGGL(k_widen_u32, dim3(grid_for(npos + 1, kBB)), dim3(kBB), 0, s, cnt, npos + 1, len64);
-        HIPCHK(hipGetLastError());
-        if ((rc = scan_i64(c, T, len64, push_ptr, (size_t)(npos + 1)))) return rc;
-        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(npos + 1), s));
-        int32_t *unsorted = nullptr;
-        if ((rc = T.get(nnz > 0 ? nnz : 1, &unsorted))) return rc;
-        hipLaunchKernelGGL(k_push_fill, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, rm_ptr, rm_col, n, push_ptr, cnt,
-                           unsorted);
-        HIPCHK(hipGetLastError());
-        if (nnz > 0) {
-            size_t bytes = 0;
-            ROCCHK(rocprim::segmented_radix_sort_keys(nullptr, bytes, unsorted, push_col, (unsigned int)nnz,
-                                                      (unsigned int)npos, push_ptr, push_ptr + 1, 0, 32, s));
-            unsigned char *tmp = nullptr;
-            if ((rc = T.get(bytes ? bytes : 1, &tmp))) return rc;
-            ROCCHK(rocprim::segmented_radix_sort_keys(tmp, bytes, unsorted, push_col, (unsigned int)nnz,
-                                                      (unsigned int)npos, push_ptr, push_ptr + 1, 0, 32, s));
-        }
+        if ((rc = transpose_by_sort(c, T, rm_ptr, rm_col, n, (int32_t)npos, nnz, push_ptr, push_col))) return rc;
     }
     HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipEventSynchronize(ev[2]));
@@ -981,18 +997,6 @@ extern "C" int albd_transpose_csr(albd_ctx *c, const int32_t *rp32, const int32_
         *bad = 1;
         return 0;
     }
-    // W's row lengths are A's column counts
-    unsigned int *cnt = nullptr;
-    int64_t *len64 = nullptr;
-    if ((rc = T.get(n + 1, &cnt)) || (rc = T.get(n + 1, &len64))) return rc;
-    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(n + 1), s));
-    if (nnz > 0) {
-        hipLaunchKernelGGL(k_hist, dim3(grid_for(nnz, kBB * 8)), dim3(kBB), 0, s, col, nnz, cnt);
-        HIPCHK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_widen_u32, dim3(grid_for((int64_t)n + 1, kBB)), dim3(kBB), 0, s, cnt, (int64_t)n + 1,
-                       len64);
-    HIPCHK(hipGetLastError());
     int64_t *wp = nullptr;
     int32_t *wc = nullptr;
     if ((rc = albd_malloc(c, sizeof(int64_t) * (size_t)(n + 1), (void **)&wp))) return rc;
@@ -1013,23 +1017,8 @@ extern "C" int albd_transpose_csr(albd_ctx *c, const int32_t *rp32, const int32_
             }
         }
     } out{c, wp, wc};
-    if ((rc = scan_i64(c, T, len64, wp, (size_t)n + 1))) return rc;
-    if (nnz > 0) {
-        // each entry (r, c) of A lands in W's row c, in any order; the rows
-        // are then sorted, which lists r ascending as a row-by-row scatter would
-        int32_t *unsorted = nullptr;
-        if ((rc = T.get(nnz, &unsorted))) return rc;
-        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned int) * (size_t)(n + 1), s));
-        hipLaunchKernelGGL(k_push_fill, dim3(grid_for(n, kBB)), dim3(kBB), 0, s, rp64, col, n, wp, cnt, unsorted);
-        HIPCHK(hipGetLastError());
-        size_t bytes = 0;
-        ROCCHK(rocprim::segmented_radix_sort_keys(nullptr, bytes, unsorted, wc, (unsigned int)nnz, (unsigned int)n,
-                                                  wp, wp + 1, 0, 32, s));
-        unsigned char *tmp = nullptr;
-        if ((rc = T.get(bytes ? bytes : 1, &tmp))) return rc;
-        ROCCHK(rocprim::segmented_radix_sort_keys(tmp, bytes, unsorted, wc, (unsigned int)nnz, (unsigned int)n, wp,
-                                                  wp + 1, 0, 32, s));
-    }
+    // W's row c lists the rows r of A's entries (r, c), ascending
+    if ((rc = transpose_by_sort(c, T, rp64, col, n, n, nnz, wp, wc))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     out.keep = true;
     *wp_out = wp;
