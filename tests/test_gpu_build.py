@@ -187,6 +187,36 @@ def test_rows_longer_than_the_long_row_threshold(ctx, directed):
         dg.close()
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_heavy_row_groups(ctx, directed):
+    """Every 64-position group holds more than 2 048 entries, so the row-major
+    copy goes through the chunked pass for heavy groups; C = 64 takes the
+    LDS-tiled column pass, the other heights the per-column one; directed,
+    the push CSR comes from the sort-based transpose."""
+    g = A.Graph.er(8192, 200.0, seed=71, perm_seed=72)
+    rp, col = g.row_ptr, g.col
+    if directed:
+        # each row keeps the entries an asymmetric hash picks: about half
+        rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(rp))
+        keep = (rows * 2654435761 + col.astype(np.int64) * 40503) % 7 < 4
+        rp = np.zeros(g.n + 1, np.int64)
+        np.cumsum(np.bincount(rows[keep], minlength=g.n), out=rp[1:])
+        col = g.col[keep].astype(np.int32)
+        g = A.Graph.from_arrays(rp, col, symmetric=False)
+    # the groups of C = 64, sigma = 4096: rows by length within each window
+    lens = -np.sort(-np.diff(rp).reshape(-1, 4096), axis=1)
+    assert lens.reshape(-1, 64).sum(axis=1).min() > 2048
+    for C_, sigma, pad in [(64, 4096, 1), (64, 64, 16), (32, 256, 1), (128, g.n, 32)]:
+        dg = assert_same_layout(ctx, g, C_, sigma, pad)
+        if C_ == 64 and pad == 1:
+            for root in g.pick_roots(2, 73):
+                o = dg.bfs(int(root))
+                lv, pa = O.bfs_queue(rp, col, int(root))
+                np.testing.assert_array_equal(o["level"], lv)
+                np.testing.assert_array_equal(o["parent"], pa)
+        dg.close()
+
+
 def test_build_errors(ctx):
     g = A.Graph.rmat(8, 4, seed=105, perm_seed=106)
     for C_, sigma, pad in [(48, 4096, 1), (64, 100, 1), (64, 4096, 7)]:
