@@ -812,7 +812,11 @@ __global__ __launch_bounds__(kQBlock) void k_words_to_queue(albd_graph g, albd_s
 // global access is a coalesced stream.  LDS: 8 bytes per window row.
 constexpr int kMaxWin = 8192;
 
-__global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, albd_state st, int if_done)
+// 512-thread blocks: a window's LDS arrays (32 KB at 4096 rows) allow four
+// blocks per CU, which then hold the CU's 32 wave slots
+constexpr int kVoBlock = 512;
+
+__global__ __launch_bounds__(kVoBlock) void k_to_vertex_order_win(albd_graph g, albd_state st, int if_done)
 {
     // queued before the host knew the BFS had ended: run only if the last
     // select kernel found the frontier empty
@@ -828,16 +832,16 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
         // offset and lp (a window's empty rows, its last positions, are
         // never reached unless one is the root: 43 % of the positions of an
         // RMAT s22 graph, whose lines are then not read)
-        for (int32_t i = threadIdx.x; i < len; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < len; i += kVoBlock) {
             s_level[i] = -1;
             s_parent[i] = -1;
         }
         __syncthreads();
-        for (int32_t i0 = threadIdx.x; i0 < len; i0 += kBlock * kV) {
+        for (int32_t i0 = threadIdx.x; i0 < len; i0 += kVoBlock * kV) {
             bool reached[kV];
 #pragma unroll
             for (int k = 0; k < kV; ++k) {
-                const int32_t i = i0 + k * kBlock;
+                const int32_t i = i0 + k * kVoBlock;
                 const int64_t p = lo + (i < len ? i : 0);
                 reached[k] = i < len && ((st.visited[p >> 6] >> (p & 63)) & 1ull);
             }
@@ -845,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
             unsigned long long x[kV];
 #pragma unroll
             for (int k = 0; k < kV; ++k) {
-                const int64_t p = lo + i0 + k * kBlock;
+                const int64_t p = lo + i0 + k * kVoBlock;
                 o[k] = reached[k] ? (int32_t)g.row_off[p] : 0;
                 x[k] = reached[k] ? st.lp[p] : 0ull;
             }
@@ -858,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_to_vertex_order_win(albd_graph g, al
             }
         }
         __syncthreads();
-        for (int32_t i = threadIdx.x; i < len; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < len; i += kVoBlock) {
             st.level[lo + i] = s_level[i];
             st.parent[lo + i] = s_parent[i];
         }
@@ -1406,7 +1410,7 @@ int albd_to_vertex_order(albd_ctx *c, const albd_graph *g, const albd_state *st,
         int64_t blocks = ((int64_t)g->n + g->win - 1) / g->win;
         int64_t cap = (int64_t)albd_num_cus(c) * 16;
         hipLaunchKernelGGL(k_to_vertex_order_win,
-                           dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))), dim3(kBlock),
+                           dim3((unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks))), dim3(kVoBlock),
                            sizeof(int32_t) * 2 * (size_t)g->win, albd_stream(c), *g, *st, if_done);
         HIPCHK(hipGetLastError());
         return 0;
